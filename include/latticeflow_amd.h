@@ -1,0 +1,128 @@
+/*
+ * latticeflow_amd.h -- C ABI of the MI355X lattice min-cut library.
+ *
+ * Plain C types only (pointers, sizes, int codes).  Every entry point returns
+ * LF_OK (0) or a negative LF_ERR_* code; the message of the last failure on the
+ * calling thread is available from lf_last_error().
+ *
+ * Each declaration names the reference interface it stands in for
+ * (hoytak/latticeflow; paths relative to the reference root).
+ */
+#ifndef LATTICEFLOW_AMD_H
+#define LATTICEFLOW_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define LF_OK 0
+#define LF_ERR_BAD_NAME -1       /* BadSolverOrKernelName (solver/interface.hpp:152-168) */
+#define LF_ERR_BAD_ARG -2        /* ValueError in pylatticeflow.pyx:157-267 */
+#define LF_ERR_NONSUBMODULAR -3  /* E00 + E11 > E01 + E10 (assert in network_flow_node.hpp:450) */
+#define LF_ERR_OVERFLOW -4       /* capacities do not fit the requested cap_bytes */
+#define LF_ERR_DEVICE -5         /* HIP runtime failure / no GPU */
+#define LF_ERR_STATE -6          /* call order (e.g. getCut before run) */
+#define LF_ERR_UNSUPPORTED -7    /* kernel/solver combination without a device path */
+#define LF_ERR_OUT_OF_RANGE -8   /* std::out_of_range (solver/tv/tv_solver.hpp:202-206) */
+
+/* which cut lf_get_cut returns */
+#define LF_CUT_SMIN 0 /* minimal source side: residual reachability from excess (canonical) */
+#define LF_CUT_SMAX 1 /* maximal source side: nodes that cannot reach a sink */
+
+typedef struct lf_handle lf_handle;
+
+/* Run-time tuning of the device solver (chosen by the autotuner when zero). */
+typedef struct {
+  int32_t rounds_per_relabel; /* push/relabel rounds between exact global relabels */
+  int32_t local_iters;        /* push/relabel iterations per tile per round, in LDS */
+  int32_t max_cycles;         /* cap on global-relabel cycles (0 = none) */
+  int32_t reserved;
+} lf_tuning;
+
+/* Solver statistics of the last lf_run. */
+typedef struct {
+  int64_t cycles;        /* global relabels */
+  int64_t rounds;        /* push/relabel tile rounds */
+  int64_t bfs_launches;  /* tile BFS launches over all global relabels */
+  int64_t flood_launches;/* S_min flood launches */
+  int64_t tiles_pushed;  /* tiles processed over all rounds */
+  double seconds;        /* wall time of the solve (device-synchronised) */
+  double push_seconds;   /* time in push/relabel rounds */
+  double bfs_seconds;    /* time in global relabels */
+} lf_stats;
+
+/* Message of the last failure on this thread ("" when none). */
+const char *lf_last_error(void);
+
+/* ---- names and kernels: solver/interface.hpp:185-190 --------------------- */
+/* isValidSolver / isValidKernel: 1 or 0. */
+int lf_is_valid_solver(const char *name);
+int lf_is_valid_kernel(const char *name);
+/* kernelDimension (interface.hpp:188); < 0 on a bad name. */
+int lf_kernel_dimension(const char *kernel);
+/* validKernelNames / validSolverNames (interface.hpp:189-190): writes a
+ * '\n'-separated list into buf (NUL-terminated, truncated to buflen) and
+ * returns the full length needed. */
+int lf_valid_kernel_names(char *buf, size_t buflen);
+int lf_valid_solver_names(char *buf, size_t buflen);
+/* Number of directions k of a kernel, and its offsets in generator order
+ * (k * ndim ints) and geocut weights (k doubles; 1.0 when not applicable). */
+int lf_kernel_size(const char *kernel);
+int lf_kernel_offsets(const char *kernel, int32_t *offsets, double *geocut_weights);
+
+/* ---- lattice handle: getLatticeFlowInterface (interface.hpp:171-183) ----- */
+/* solver: "LatticeEnergyMinimizer" (binary min-cut) or "LatticeLevelReductions"
+ * (parametric divide-and-conquer, pf_policies.hpp:30-46).  cap_bytes: 4 or 8,
+ * the integer width of device capacities (8 = the reference default `long`). */
+int lf_create(lf_handle **out, const char *solver, const char *kernel, int ndim,
+              const int64_t *shape, int cap_bytes);
+int lf_destroy(lf_handle *h);
+
+/* nDimensions / nNodes / nFillingEdges / edgeDeltas (interface.hpp:24-45) */
+int lf_n_dimensions(const lf_handle *h);
+int64_t lf_n_nodes(const lf_handle *h);
+int lf_n_filling_edges(const lf_handle *h);
+/* edge_deltas[i*ndim + j], i < nFillingEdges (kernellattice.hpp:61-96 order) */
+int lf_edge_deltas(const lf_handle *h, int32_t *edge_deltas);
+/* all k directions in lattice order (first k/2 = edgeDeltas, then reversed) */
+int lf_directions(const lf_handle *h, int32_t *offsets);
+
+/* addEnergyPotentials(long*, long*) (interface.hpp:61, direct_interface.hpp:65-95).
+ * unary[2*n + {0,1}] = E0, E1; binary[n*4*(k/2) + 4*e + {0..3}] = E00,E01,E10,E11.
+ * Host pointers; accumulates like the reference (a second call adds). */
+int lf_add_energy_potentials(lf_handle *h, const int64_t *unary, const int64_t *binary);
+/* The same from device-resident arrays (no host copy), ordered on `stream`
+ * (a hipStream_t; NULL = the handle's stream). */
+int lf_add_energy_potentials_device(lf_handle *h, const int64_t *d_unary,
+                                    const int64_t *d_binary, void *stream);
+/* Clear all accumulated energies (a fresh lattice). */
+int lf_reset_energies(lf_handle *h);
+
+/* run() (interface.hpp:113, direct_interface.hpp:49-58).  Solves from the
+ * accumulated energies; tuning may be NULL (autotuned defaults). */
+int lf_set_tuning(lf_handle *h, const lf_tuning *tuning);
+int lf_run(lf_handle *h);
+/* runTime() (interface.hpp:115): wall seconds of the last lf_run. */
+int lf_run_time(const lf_handle *h, double *seconds);
+int lf_get_stats(const lf_handle *h, lf_stats *stats);
+
+/* getCut() (interface.hpp:123, direct_interface.hpp:97-106): out[n] = 1 when
+ * node n (C order over the unpadded shape) is on the source ("on") side.
+ * LatticeEnergyMinimizer: `which` selects S_min or S_max.
+ * LatticeLevelReductions: the level-reduction cut {r < 0} (pf_solver.hpp:181-182). */
+int lf_get_cut(const lf_handle *h, int32_t *out, int which);
+/* The max-flow value of the last run in raw (doubled) units
+ * (network_flow_node.hpp:234-240); the reference computes it but does not expose it. */
+int lf_get_flow(const lf_handle *h, int64_t *flow);
+/* Device pointer variant of getCut for device-resident consumers. */
+int lf_get_cut_device(const lf_handle *h, int32_t *d_out, int which, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LATTICEFLOW_AMD_H */

@@ -1,0 +1,9 @@
+"""MI355X-native lattice push-relabel min-cut (hoytak/latticeflow's hot path).
+
+The product is the HIP library ``liblatticeflow_amd.so`` behind the C ABI in
+``include/latticeflow_amd.h``; this package holds its Python host mirror
+(``pylatticeflow``), the synthetic workload generators and the build script.
+"""
+from .pylatticeflow import LatticeFlow, getKernelDimension, validKernelNames, validSolverNames  # noqa: F401
+
+__all__ = ["LatticeFlow", "getKernelDimension", "validKernelNames", "validSolverNames"]

@@ -1,0 +1,326 @@
+// C ABI of the library (declared in include/latticeflow_amd.h).
+//
+// Stands in for the type-erased reference API: the factory
+// getLatticeFlowInterface and name helpers (solver/interface.hpp:171-190,
+// generated factory install/templates/factory/file.cpp:83-124) and
+// LatticeFlowDirectInterface (solver/interfacing/direct_interface.hpp:11-108).
+// Errors are returned as LF_ERR_* codes; the C++ and Python hosts turn them
+// back into BadSolverOrKernelName / ValueError / std::out_of_range.
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/latticeflow_amd.h"
+#include "lf_engine.hpp"
+#include "lf_kernels.hpp"
+#include "lf_levels.hpp"
+
+using namespace lfamd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+const char* const kSolvers[] = {"LatticeEnergyMinimizer", "LatticeLevelReductions"};
+
+bool valid_solver(const std::string& s) {
+  for (const char* n : kSolvers)
+    if (s == n) return true;
+  return false;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return LF_OK;
+  } catch (const LfError& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "out of host memory";
+    return LF_ERR_DEVICE;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return LF_ERR_DEVICE;
+  }
+}
+
+int copy_list(const std::vector<std::string>& names, char* buf, size_t buflen) {
+  std::string s;
+  for (size_t i = 0; i < names.size(); ++i) {
+    if (i) s += '\n';
+    s += names[i];
+  }
+  if (buf && buflen) {
+    size_t n = std::min(buflen - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return (int)s.size();
+}
+
+}  // namespace
+
+struct lf_handle {
+  std::string solver, kernel;
+  int ndim = 0;
+  std::vector<int64_t> shape;
+  int64_t n = 0;
+  int k = 0;
+  int cap_bytes = 8;
+  std::vector<int> offs;      // k * ndim, lattice order
+  std::vector<double> geocut; // k
+  LatticeGeom geom;
+  std::unique_ptr<MinCutEngine> eng;
+  std::unique_ptr<LevelSolver> levels;  // LatticeLevelReductions
+  lf_tuning tuning{};
+  double run_time = 0;
+};
+
+extern "C" {
+
+const char* lf_last_error(void) { return g_last_error.c_str(); }
+
+int lf_is_valid_solver(const char* name) { return name && valid_solver(name) ? 1 : 0; }
+
+int lf_is_valid_kernel(const char* name) { return name && find_kernel(name) ? 1 : 0; }
+
+int lf_kernel_dimension(const char* kernel) {
+  const KernelDef* k = kernel ? find_kernel(kernel) : nullptr;
+  if (!k) {
+    g_last_error = std::string("Bad Kernel or Solver Name: ") + (kernel ? kernel : "(null)");
+    return LF_ERR_BAD_NAME;
+  }
+  return k->ndim;
+}
+
+int lf_valid_kernel_names(char* buf, size_t buflen) {
+  std::vector<std::string> names;
+  for (const auto& kv : kernel_registry()) names.push_back(kv.first);
+  return copy_list(names, buf, buflen);
+}
+
+int lf_valid_solver_names(char* buf, size_t buflen) {
+  std::vector<std::string> names(std::begin(kSolvers), std::end(kSolvers));
+  return copy_list(names, buf, buflen);
+}
+
+int lf_kernel_size(const char* kernel) {
+  const KernelDef* k = kernel ? find_kernel(kernel) : nullptr;
+  if (!k) {
+    g_last_error = std::string("Bad Kernel or Solver Name: ") + (kernel ? kernel : "(null)");
+    return LF_ERR_BAD_NAME;
+  }
+  return (int)k->edges.size();
+}
+
+int lf_kernel_offsets(const char* kernel, int32_t* offsets, double* w) {
+  return guarded([&] {
+    const KernelDef* k = kernel ? find_kernel(kernel) : nullptr;
+    if (!k) fail(LF_ERR_BAD_NAME, std::string("Bad Kernel or Solver Name: ") + (kernel ? kernel : "(null)"));
+    for (size_t i = 0; i < k->edges.size(); ++i) {
+      for (int d = 0; d < k->ndim; ++d)
+        if (offsets) offsets[i * k->ndim + d] = k->edges[i][d];
+      if (w) w[i] = k->geocut_w[i];
+    }
+  });
+}
+
+int lf_create(lf_handle** out, const char* solver, const char* kernel, int ndim, const int64_t* shape,
+              int cap_bytes) {
+  if (out) *out = nullptr;
+  return guarded([&] {
+    if (!out || !solver || !kernel || !shape) fail(LF_ERR_BAD_ARG, "null argument");
+    if (!valid_solver(solver)) fail(LF_ERR_BAD_NAME, std::string("Bad Kernel or Solver Name: ") + solver);
+    const KernelDef* kd = find_kernel(kernel);
+    if (!kd) fail(LF_ERR_BAD_NAME, std::string("Bad Kernel or Solver Name: ") + kernel);
+    if (ndim != kd->ndim)
+      fail(LF_ERR_BAD_ARG, "Kernel '" + std::string(kernel) + "' is " + std::to_string(kd->ndim) +
+                               "-dimensional; a shape of " + std::to_string(ndim) + " dimensions was provided.");
+    if (cap_bytes != 4 && cap_bytes != 8) fail(LF_ERR_BAD_ARG, "cap_bytes must be 4 or 8");
+    std::unique_ptr<lf_handle> h(new lf_handle);
+    h->solver = solver;
+    h->kernel = kernel;
+    h->ndim = ndim;
+    h->shape.assign(shape, shape + ndim);
+    h->n = 1;
+    for (int d = 0; d < ndim; ++d) {
+      if (shape[d] <= 0) fail(LF_ERR_BAD_ARG, "every dimension must be positive");
+      if (shape[d] > (1LL << 30)) fail(LF_ERR_BAD_ARG, "dimension too large");
+      h->n *= shape[d];
+    }
+    h->k = (int)kd->edges.size();
+    h->cap_bytes = cap_bytes;
+    if (!lattice_edge_order(*kd, h->shape, h->offs, h->geocut))
+      fail(LF_ERR_BAD_ARG, "kernel is not symmetric on this lattice");
+    LatticeGeom& g = h->geom;
+    g.ndim = ndim;
+    g.k = h->k;
+    g.D = ndim == 3 ? shape[0] : 1;
+    g.H = ndim == 3 ? shape[1] : (ndim == 2 ? shape[0] : 1);
+    g.W = shape[ndim - 1];
+    for (int i = 0; i < h->k; ++i) {
+      const int* o = &h->offs[i * ndim];
+      g.dz.push_back(ndim == 3 ? o[0] : 0);
+      g.dy.push_back(ndim == 3 ? o[1] : (ndim == 2 ? o[0] : 0));
+      g.dx.push_back(o[ndim - 1]);
+    }
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+      fail(LF_ERR_DEVICE, "no HIP device available");
+    h->eng = make_mincut_engine(g, cap_bytes);
+    if (h->solver == "LatticeLevelReductions") h->levels = make_level_solver(g, h->eng.get());
+    *out = h.release();
+  });
+}
+
+int lf_destroy(lf_handle* h) {
+  return guarded([&] { delete h; });
+}
+
+int lf_n_dimensions(const lf_handle* h) { return h ? h->ndim : LF_ERR_BAD_ARG; }
+int64_t lf_n_nodes(const lf_handle* h) { return h ? h->n : LF_ERR_BAD_ARG; }
+int lf_n_filling_edges(const lf_handle* h) { return h ? h->k / 2 : LF_ERR_BAD_ARG; }
+
+int lf_edge_deltas(const lf_handle* h, int32_t* ed) {
+  return guarded([&] {
+    if (!h || !ed) fail(LF_ERR_BAD_ARG, "null argument");
+    for (int i = 0; i < h->k / 2 * h->ndim; ++i) ed[i] = h->offs[i];
+  });
+}
+
+int lf_directions(const lf_handle* h, int32_t* offs) {
+  return guarded([&] {
+    if (!h || !offs) fail(LF_ERR_BAD_ARG, "null argument");
+    for (int i = 0; i < h->k * h->ndim; ++i) offs[i] = h->offs[i];
+  });
+}
+
+int lf_add_energy_potentials_device(lf_handle* h, const int64_t* du, const int64_t* db, void* stream) {
+  return guarded([&] {
+    if (!h || !du || !db) fail(LF_ERR_BAD_ARG, "null argument");
+    h->eng->add_energies_device(du, db, reinterpret_cast<hipStream_t>(stream));
+    if (h->levels) h->levels->invalidate();
+  });
+}
+
+int lf_add_energy_potentials(lf_handle* h, const int64_t* unary, const int64_t* binary) {
+  return guarded([&] {
+    if (!h || !unary || !binary) fail(LF_ERR_BAD_ARG, "null argument");
+    const size_t nu = (size_t)h->n * 2, nb = (size_t)h->n * (h->k / 2) * 4;
+    int64_t *du = nullptr, *db = nullptr;
+    hipStream_t s = h->eng->stream();
+    LF_HIP(hipMallocAsync((void**)&du, nu * sizeof(int64_t), s));
+    LF_HIP(hipMallocAsync((void**)&db, nb * sizeof(int64_t), s));
+    struct Free {
+      int64_t *a, *b;
+      hipStream_t s;
+      ~Free() {
+        hipFreeAsync(a, s);
+        hipFreeAsync(b, s);
+        hipStreamSynchronize(s);
+      }
+    } fr{du, db, s};
+    LF_HIP(hipMemcpyAsync(du, unary, nu * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    LF_HIP(hipMemcpyAsync(db, binary, nb * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    h->eng->add_energies_device(du, db, s);
+    if (h->levels) h->levels->invalidate();
+  });
+}
+
+int lf_reset_energies(lf_handle* h) {
+  return guarded([&] {
+    if (!h) fail(LF_ERR_BAD_ARG, "null argument");
+    h->eng->reset_energies();
+    if (h->levels) h->levels->invalidate();
+  });
+}
+
+int lf_set_tuning(lf_handle* h, const lf_tuning* t) {
+  return guarded([&] {
+    if (!h) fail(LF_ERR_BAD_ARG, "null argument");
+    h->tuning = t ? *t : lf_tuning{};
+  });
+}
+
+int lf_run(lf_handle* h) {
+  return guarded([&] {
+    if (!h) fail(LF_ERR_BAD_ARG, "null argument");
+    const auto t0 = std::chrono::steady_clock::now();
+    if (h->levels) h->levels->run(h->tuning);
+    else h->eng->run(h->tuning);
+    h->run_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int lf_run_time(const lf_handle* h, double* seconds) {
+  return guarded([&] {
+    if (!h || !seconds) fail(LF_ERR_BAD_ARG, "null argument");
+    *seconds = h->run_time;
+  });
+}
+
+int lf_get_stats(const lf_handle* h, lf_stats* st) {
+  return guarded([&] {
+    if (!h || !st) fail(LF_ERR_BAD_ARG, "null argument");
+    *st = h->levels ? h->levels->stats : h->eng->stats;
+  });
+}
+
+int lf_get_cut_device(const lf_handle* h, int32_t* d_out, int which, void* stream) {
+  return guarded([&] {
+    if (!h || !d_out) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->levels) {
+      if (!h->levels->solved()) fail(LF_ERR_STATE, "run() has not been called");
+      h->levels->cut_device(d_out, reinterpret_cast<hipStream_t>(stream));
+      return;
+    }
+    if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
+    if (which != LF_CUT_SMIN && which != LF_CUT_SMAX) fail(LF_ERR_BAD_ARG, "bad cut selector");
+    h->eng->cut_device(d_out, which, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int lf_get_cut(const lf_handle* h, int32_t* out, int which) {
+  return guarded([&] {
+    if (!h || !out) fail(LF_ERR_BAD_ARG, "null argument");
+    hipStream_t s = h->eng->stream();
+    int32_t* d = nullptr;
+    LF_HIP(hipMallocAsync((void**)&d, sizeof(int32_t) * h->n, s));
+    struct Free {
+      int32_t* d;
+      hipStream_t s;
+      ~Free() {
+        hipFreeAsync(d, s);
+        hipStreamSynchronize(s);
+      }
+    } fr{d, s};
+    if (h->levels) {
+      if (!h->levels->solved()) fail(LF_ERR_STATE, "run() has not been called");
+      h->levels->cut_device(d, s);
+    } else {
+      if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
+      if (which != LF_CUT_SMIN && which != LF_CUT_SMAX) fail(LF_ERR_BAD_ARG, "bad cut selector");
+      h->eng->cut_device(d, which, s);
+    }
+    LF_HIP(hipMemcpyAsync(out, d, sizeof(int32_t) * h->n, hipMemcpyDeviceToHost, s));
+    LF_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int lf_get_flow(const lf_handle* h, int64_t* flow) {
+  return guarded([&] {
+    if (!h || !flow) fail(LF_ERR_BAD_ARG, "null argument");
+    if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
+    *flow = h->eng->flow;
+  });
+}
+
+}  // extern "C"

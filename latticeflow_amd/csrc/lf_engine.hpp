@@ -1,0 +1,59 @@
+// Device min-cut engine interface (host side).  The kernels live in lf_mincut.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/latticeflow_amd.h"
+
+namespace lfamd {
+
+// Thrown inside the library and turned into an LF_ERR_* code at the C ABI.
+struct LfError {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw LfError{code, msg}; }
+
+#define LF_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      ::lfamd::fail(LF_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Lattice geometry handed to the engine: every lattice is viewed as (D, H, W)
+// (1-D: (1,1,n); 2-D: (1,n0,n1); 3-D: (n0,n1,n2)) with directions in the
+// reference lattice order, entry i + k/2 the reverse of entry i.
+struct LatticeGeom {
+  int ndim = 0;
+  int64_t D = 1, H = 1, W = 1;
+  int k = 0;
+  std::vector<int> dz, dy, dx;
+  int64_t n() const { return D * H * W; }
+};
+
+class MinCutEngine {
+ public:
+  virtual ~MinCutEngine() {}
+  // Accumulate energies (reference layout, device pointers) into the base network.
+  virtual void add_energies_device(const int64_t* d_unary, const int64_t* d_binary, hipStream_t s) = 0;
+  virtual void reset_energies() = 0;
+  // Solve the binary min-cut of the accumulated network.
+  virtual void run(const lf_tuning& tuning) = 0;
+  // Cut of the last solve (LF_CUT_SMIN / LF_CUT_SMAX), as int32 0/1 into device memory.
+  virtual void cut_device(int32_t* d_out, int which, hipStream_t s) = 0;
+  virtual hipStream_t stream() const = 0;
+  int64_t flow = 0;
+  bool solved = false;
+  lf_stats stats{};
+};
+
+std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_bytes);
+
+}  // namespace lfamd

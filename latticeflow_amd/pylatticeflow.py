@@ -1,0 +1,231 @@
+"""Python API of the reference's ``pylatticeflow`` extension, on the MI355X library.
+
+Mirrors ``extensions/python/pylatticeflow.pyx`` of hoytak/latticeflow:
+
+* ``getKernelDimension(kernel)``                      -- pyx:71-80
+* ``LatticeFlow(kernel, solver, shape, numerical_precision=1e-6)`` -- pyx:82-302
+  with ``ndim / shape / size / nFillingEdges / edges``, ``addEnergy``,
+  ``run``, ``runTime``, ``getCut``.
+
+Same argument meaning and the same ``ValueError`` messages for malformed
+energies.  Additions, all keyword-only or new methods: ``cap_bytes`` (device
+integer width; 8 = the reference's ``long``), ``getCut(which=...)`` to choose
+S_min (default) or S_max, ``flow()`` (raw max-flow value), ``stats()`` and
+``setTuning``.  Every call goes through the C ABI to the HIP library; there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _names(fn):
+    n = fn(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    fn(buf, n + 1)
+    return [s for s in buf.value.decode().split("\n") if s]
+
+
+def validKernelNames():
+    return _names(lib().lf_valid_kernel_names)
+
+
+def validSolverNames():
+    return _names(lib().lf_valid_solver_names)
+
+
+def getKernelDimension(kernel: str) -> int:
+    """pyx:71-80: ValueError listing the valid kernels on a bad name."""
+    if not lib().lf_is_valid_kernel(kernel.encode()):
+        raise ValueError("Kernel '%s' not a valid kernel name!  Valid kernels are: %s."
+                         % (kernel, ", ".join(validKernelNames())))
+    return lib().lf_kernel_dimension(kernel.encode())
+
+
+class LatticeFlow(object):
+    """An interface to the lattice flow solvers (pyx:82-302)."""
+
+    def __init__(self, kernel, solver, _dimensions, numerical_precision=1e-6, cap_bytes=8):
+        if not lib().lf_is_valid_solver(solver.encode()):
+            raise ValueError("'%s' not a valid solver name!  Valid solvers are: %s."
+                             % (solver, ", ".join(validSolverNames())))
+        dim = getKernelDimension(kernel)
+        self._dimensions = tuple(int(d) for d in _dimensions)
+        self.n_nodes = int(np.prod(self._dimensions))
+        if len(self._dimensions) != dim:
+            raise ValueError("Kernel '%s' is %d-dimensional; a shape of %d dimensions was provided."
+                             % (kernel, dim, len(self._dimensions)))
+        self.numerical_precision_factor = 1.0 / numerical_precision
+        shape = (ctypes.c_int64 * dim)(*self._dimensions)
+        h = ctypes.c_void_p()
+        check(lib().lf_create(ctypes.byref(h), solver.encode(), kernel.encode(), dim, shape, int(cap_bytes)))
+        self._h = h
+        self._kernel = kernel
+        self._solver = solver
+        n_edges = lib().lf_n_filling_edges(h)
+        e = np.empty((n_edges, dim), dtype=np.int32)
+        check(lib().lf_edge_deltas(h, e.ctypes.data))
+        self._edges = e
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().lf_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def ndim(self):
+        return len(self._dimensions)
+
+    @property
+    def shape(self):
+        return self._dimensions
+
+    @property
+    def size(self):
+        return self.n_nodes
+
+    @property
+    def nFillingEdges(self):
+        return int(self._edges.shape[0])
+
+    @property
+    def edges(self):
+        return self._edges
+
+    def directions(self):
+        """All k directions in lattice order (edges, then their reverses)."""
+        k = 2 * self.nFillingEdges
+        d = np.empty((k, self.ndim), dtype=np.int32)
+        check(lib().lf_directions(self._h, d.ctypes.data))
+        return d
+
+    def addEnergy(self, unary_energy, pairwise_energy):
+        """pyx:157-267: scale by 1/numerical_precision, round to int64, validate, add."""
+        nf = self.numerical_precision_factor
+        unary_energy = np.ascontiguousarray(np.round(nf * np.asarray(unary_energy)), dtype=np.int64)
+        pairwise_energy = np.ascontiguousarray(np.round(nf * np.asarray(pairwise_energy)), dtype=np.int64)
+        self._add_int_energy(unary_energy, pairwise_energy)
+
+    def addEnergyInt(self, unary_energy, pairwise_energy):
+        """Add already-integral energies (no numerical_precision scaling)."""
+        self._add_int_energy(np.ascontiguousarray(unary_energy, dtype=np.int64),
+                             np.ascontiguousarray(pairwise_energy, dtype=np.int64))
+
+    def _add_int_energy(self, unary_energy, pairwise_energy):
+        unary_energy_error = (
+            "Unary energy must be either a 2d array with first dimension "
+            "matching the number of nodes and second dimension containing E0 and E1, "
+            "or (#dimensions + 1) with the first matching the lattice coordinates and "
+            "the last containing E0 and E1.")
+        if unary_energy.ndim == 2:
+            if unary_energy.shape[0] != self.n_nodes:
+                raise ValueError(unary_energy_error + (" (# nodes = %d; # unary_array_nodes = %d)"
+                                                       % (self.n_nodes, unary_energy.shape[0])))
+            if unary_energy.shape[1] != 2:
+                raise ValueError(unary_energy_error + (" (Second dimension has shape %d; 2 required.)"
+                                                       % unary_energy.shape[1]))
+            unary_a = unary_energy
+        elif unary_energy.ndim == len(self._dimensions) + 1:
+            for i in range(len(self._dimensions)):
+                if unary_energy.shape[i] != self._dimensions[i]:
+                    raise ValueError(unary_energy_error + (" (dimension %d = %d; unary_array shape is %d)"
+                                                           % (i, self._dimensions[i], unary_energy.shape[i])))
+            if unary_energy.shape[-1] != 2:
+                raise ValueError(unary_energy_error + (" (Last dimension has shape %d; 2 required.)"
+                                                       % unary_energy.shape[-1]))
+            unary_a = unary_energy.reshape(-1, 2)
+        else:
+            raise ValueError(unary_energy_error + " (wrong number of dimensions)")
+
+        pairwise_energy_error = (
+            "Pairwise energy must be either: (1) a 3d array with first dimension "
+            "matching the number of nodes, second dimension containing the edge direction output, "
+            "and third dimension containing E00, E01, E10, and E11 (with E01 being energy of "
+            "source node off and dest node on.), or (2) a (#dimensions "
+            "+ 2) array with last two dimensions being the same and first dimensions corresponding "
+            "to the lattice coordinates. ")
+        bad_edge_error = (pairwise_energy_error +
+                          " (wrong number of edges specified in second-to-last dimension; got %d, needed %d).")
+        bad_last_error = pairwise_energy_error + " (wrong number of pairwise energy terms; needed 4, got %d."
+        ne = self.nFillingEdges
+        if pairwise_energy.ndim == 3:
+            if pairwise_energy.shape[0] != self.n_nodes:
+                raise ValueError(pairwise_energy_error + (" (# nodes = %d; # nodes in pairwise_energy = %d)"
+                                                          % (self.n_nodes, pairwise_energy.shape[0])))
+            if pairwise_energy.shape[1] != ne:
+                raise ValueError(bad_edge_error % (pairwise_energy.shape[1], ne))
+            if pairwise_energy.shape[2] != 4:
+                raise ValueError(bad_last_error % pairwise_energy.shape[2])
+            pairwise_a = pairwise_energy
+        elif pairwise_energy.ndim == len(self._dimensions) + 2:
+            for i in range(len(self._dimensions)):
+                if pairwise_energy.shape[i] != self._dimensions[i]:
+                    raise ValueError(pairwise_energy_error + (" (dimension %d shape is %d; pairwise_array shape is %d)"
+                                                              % (i, self._dimensions[i], pairwise_energy.shape[i])))
+            if pairwise_energy.shape[-2] != ne:
+                raise ValueError(bad_edge_error % (pairwise_energy.shape[-2], ne))
+            if pairwise_energy.shape[-1] != 4:
+                raise ValueError(bad_last_error % pairwise_energy.shape[-1])
+            pairwise_a = pairwise_energy.reshape(-1, ne, 4)
+        else:
+            raise ValueError(pairwise_energy_error + " (Array has wrong number of dimensions.)")
+        unary_a = np.ascontiguousarray(unary_a, dtype=np.int64)
+        pairwise_a = np.ascontiguousarray(pairwise_a, dtype=np.int64)
+        check(lib().lf_add_energy_potentials(self._h, unary_a.ctypes.data, pairwise_a.ctypes.data))
+
+    def addEnergyDevice(self, unary_ptr, pairwise_ptr, stream=None):
+        """Add energies already resident in device memory (int64, reference layout)."""
+        check(lib().lf_add_energy_potentials_device(self._h, ctypes.c_void_p(unary_ptr),
+                                                    ctypes.c_void_p(pairwise_ptr),
+                                                    ctypes.c_void_p(stream or 0)))
+
+    def resetEnergy(self):
+        check(lib().lf_reset_energies(self._h))
+
+    def setTuning(self, rounds_per_relabel=0, local_iters=0, max_cycles=0):
+        t = _lib.lf_tuning(int(rounds_per_relabel), int(local_iters), int(max_cycles), 0)
+        check(lib().lf_set_tuning(self._h, ctypes.byref(t)))
+
+    def run(self):
+        """Runs the specified solver (pyx:269-274)."""
+        check(lib().lf_run(self._h))
+
+    def runTime(self):
+        """Wall seconds of the last run (pyx:277-282; the reference reports CPU time)."""
+        t = ctypes.c_double()
+        check(lib().lf_run_time(self._h, ctypes.byref(t)))
+        return t.value
+
+    def stats(self):
+        s = _lib.lf_stats()
+        check(lib().lf_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def getCut(self, which="smin"):
+        """Cut as an int32 array of ``shape`` (pyx:285-302); 1 = source ("on") side.
+
+        ``which``: "smin" (minimal source side, the default) or "smax".
+        """
+        sel = {"smin": _lib.LF_CUT_SMIN, "smax": _lib.LF_CUT_SMAX}[which]
+        out = np.empty(self.n_nodes, dtype=np.int32)
+        check(lib().lf_get_cut(self._h, out.ctypes.data, sel))
+        return out.reshape(*self._dimensions)
+
+    def getCutDevice(self, out_ptr, which="smin", stream=None):
+        sel = {"smin": _lib.LF_CUT_SMIN, "smax": _lib.LF_CUT_SMAX}[which]
+        check(lib().lf_get_cut_device(self._h, ctypes.c_void_p(out_ptr), sel, ctypes.c_void_p(stream or 0)))
+
+    def flow(self):
+        """Raw (doubled-unit) max-flow value of the last run."""
+        f = ctypes.c_int64()
+        check(lib().lf_get_flow(self._h, ctypes.byref(f)))
+        return f.value

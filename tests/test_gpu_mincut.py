@@ -1,0 +1,80 @@
+"""Device min-cut parity against the CPU oracle (flow value and canonical cuts, bit-exact).
+
+Every case goes through the C ABI of liblatticeflow_amd.so (via the Python host
+mirror).  The oracle is the PRFlow restatement in oracle/lf_oracle.c, itself
+pinned against Dinic and brute force in tests/test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_energies(rng, shape, offs, cap=20, wcap=10):
+    n = int(np.prod(shape))
+    kp = len(offs) // 2
+    un = rng.integers(0, cap, (n, 2))
+    b = np.zeros((n, kp, 4), np.int64)
+    b[:, :, 0] = rng.integers(0, 3, (n, kp))
+    b[:, :, 3] = rng.integers(0, 3, (n, kp))
+    b[:, :, 1] = rng.integers(0, wcap, (n, kp)) + b[:, :, 0]
+    b[:, :, 2] = rng.integers(0, wcap, (n, kp)) + b[:, :, 3]
+    return un, b
+
+
+def _solve_gpu(kernel, shape, un, b, cap_bytes=8, tuning=None):
+    from latticeflow_amd import LatticeFlow
+    lf = LatticeFlow(kernel, "LatticeEnergyMinimizer", shape, numerical_precision=1.0, cap_bytes=cap_bytes)
+    if tuning:
+        lf.setTuning(**tuning)
+    lf.addEnergyInt(un, b)
+    lf.run()
+    return lf.flow(), lf.getCut("smin").reshape(-1), lf.getCut("smax").reshape(-1), lf
+
+
+def _check(oracle, kernel, shape, un, b, cap_bytes=8, tuning=None):
+    lfl = _solve_gpu(kernel, shape, un, b, cap_bytes, tuning)
+    offs = lfl[3].directions()
+    red, alpha = oracle.reparam(shape, offs, un, b)
+    ref = oracle.prflow(shape, offs, red, alpha)
+    flow, smin, smax, _ = lfl
+    assert flow == ref["flow"], (kernel, shape, flow, ref["flow"])
+    assert np.array_equal(smin, ref["smin"].astype(np.int32)), (kernel, shape)
+    assert np.array_equal(smax, ref["smax"].astype(np.int32)), (kernel, shape)
+    return ref
+
+
+@pytest.mark.parametrize("kernel,nd", [("Full2d_4", 2), ("Full2d_8", 2), ("Star2d_16", 2),
+                                       ("Full3d_6", 3), ("Star3d_26", 3), ("Full1d_2", 1)])
+@pytest.mark.parametrize("cap_bytes", [4, 8])
+def test_random_small(oracle, kernel, nd, cap_bytes):
+    rng = np.random.default_rng(hash((kernel, cap_bytes)) & 0xffff)
+    for trial in range(6):
+        if nd == 1:
+            shape = (int(rng.integers(1, 300)),)
+        elif nd == 2:
+            shape = tuple(int(x) for x in rng.integers(1, 90, 2))
+        else:
+            shape = tuple(int(x) for x in rng.integers(1, 20, 3))
+        offs, _ = oracle.edge_order(kernel, shape)
+        un, b = _random_energies(rng, shape, offs)
+        print("case", kernel, cap_bytes, shape, flush=True)
+        _check(oracle, kernel, shape, un, b, cap_bytes)
+
+
+def test_grid4_256_config(oracle):
+    from latticeflow_amd import workloads as W
+    w = W.grid4_256(seed=1)
+    _check(oracle, w["kernel"], w["shape"], w["unary"], w["pairwise"], cap_bytes=4)
+
+
+def test_seg8_512(oracle):
+    from latticeflow_amd import workloads as W
+    w = W.seg8(seed=5, shape=(512, 512))
+    _check(oracle, w["kernel"], w["shape"], w["unary"], w["pairwise"], cap_bytes=4)
+
+
+def test_blobs26_small(oracle):
+    from latticeflow_amd import workloads as W
+    w = W.blobs26(seed=7, shape=(40, 48, 56))
+    _check(oracle, w["kernel"], w["shape"], w["unary"], w["pairwise"], cap_bytes=8)
