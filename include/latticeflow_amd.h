@@ -46,13 +46,17 @@ typedef struct {
 /* Solver statistics of the last lf_run. */
 typedef struct {
   int64_t cycles;        /* global relabels */
-  int64_t rounds;        /* push/relabel tile rounds */
+  int64_t rounds;        /* push/relabel round launches */
   int64_t bfs_launches;  /* tile BFS launches over all global relabels */
   int64_t flood_launches;/* S_min flood launches */
-  int64_t tiles_pushed;  /* tiles processed over all rounds */
+  int64_t tiles_pushed;  /* tiles processed (not skipped) over all push launches */
+  int64_t tile_nodes;    /* nodes per tile of the push kernel */
+  int64_t bytes_per_node;/* algorithmic HBM bytes per processed node of one push launch */
+  int64_t reserved;
   double seconds;        /* wall time of the solve (device-synchronised) */
-  double push_seconds;   /* time in push/relabel rounds */
-  double bfs_seconds;    /* time in global relabels */
+  double push_seconds;   /* device time of all push launches (hipEvents on the solver stream) */
+  double bfs_seconds;    /* wall time in global relabels (BFS + delivery + counting) */
+  double flood_seconds;  /* wall time of the S_min flood */
 } lf_stats;
 
 /* Message of the last failure on this thread ("" when none). */

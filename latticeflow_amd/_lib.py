@@ -44,8 +44,10 @@ class lf_tuning(ctypes.Structure):
 class lf_stats(ctypes.Structure):
     _fields_ = [("cycles", ctypes.c_int64), ("rounds", ctypes.c_int64),
                 ("bfs_launches", ctypes.c_int64), ("flood_launches", ctypes.c_int64),
-                ("tiles_pushed", ctypes.c_int64), ("seconds", ctypes.c_double),
-                ("push_seconds", ctypes.c_double), ("bfs_seconds", ctypes.c_double)]
+                ("tiles_pushed", ctypes.c_int64), ("tile_nodes", ctypes.c_int64),
+                ("bytes_per_node", ctypes.c_int64), ("reserved", ctypes.c_int64),
+                ("seconds", ctypes.c_double), ("push_seconds", ctypes.c_double),
+                ("bfs_seconds", ctypes.c_double), ("flood_seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -74,6 +74,7 @@ struct KP {
   int* flag_in;
   int* flag_out;
   unsigned long long* counter;
+  unsigned long long* tiles;  // processed-tile counter of the push rounds
   int D, H, W;
   long long N;
   int ntx, nty, ntz, ntiles, nblocks;
@@ -94,6 +95,42 @@ __device__ inline void gadd(long long* a, long long v) {
 __device__ inline int gtake(int* a) { return atomicExch(a, 0); }
 __device__ inline long long gtake(long long* a) {
   return static_cast<long long>(atomicExch(reinterpret_cast<unsigned long long*>(a), 0ull));
+}
+
+// Streaming kernels that reduce run a fixed grid of RED_BLOCKS blocks with a
+// grid-stride loop and leave one partial per block; k_reduce folds them.  No
+// two blocks ever update the same word.
+constexpr int RED_BLOCKS = 2048;
+
+template <bool IS_MAX>
+__device__ inline long long block_reduce(long long v) {
+  __shared__ long long s_part[NTHREADS / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    const long long w = __shfl_down(v, o, 64);
+    v = IS_MAX ? (w > v ? w : v) : v + w;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) s_part[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = s_part[0];
+    for (int i = 1; i < NTHREADS / 64; ++i) v = IS_MAX ? (s_part[i] > v ? s_part[i] : v) : v + s_part[i];
+  }
+  __syncthreads();
+  return v;  // valid in thread 0
+}
+
+// out[j] = fold over b < nparts of part[b * stride + j], one block
+template <bool IS_MAX>
+__global__ __launch_bounds__(NTHREADS) void k_reduce(const long long* __restrict__ part, int nparts, int stride,
+                                                     int j, unsigned long long* out) {
+  long long v = 0;
+  for (int b = threadIdx.x; b < nparts; b += NTHREADS) {
+    const long long w = part[(long long)b * stride + j];
+    v = IS_MAX ? (w > v ? w : v) : v + w;
+  }
+  v = block_reduce<IS_MAX>(v);
+  if (threadIdx.x == 0) out[0] = (unsigned long long)v;
 }
 
 // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous run
@@ -160,6 +197,14 @@ __device__ inline void flag_neighbor_tile(const KP<T>& p, int gz, int gy, int gx
 
 // ---- the push / relabel round -----------------------------------------------
 
+// Local coordinates of tile slot t.
+template <class G>
+__device__ inline void tile_coords(int t, int& z, int& y, int& x) {
+  z = t / (G::TH * G::TW);
+  y = (t / G::TW) % G::TH;
+  x = t % G::TW;
+}
+
 template <int K, class G, typename T>
 __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   constexpr int NT = G::NT;
@@ -167,6 +212,8 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   __shared__ T s_ex[NT];
   __shared__ T s_in[NT];
   __shared__ int s_h[G::NH];
+  __shared__ unsigned short s_list[NT];
+  __shared__ int s_cnt[2];
   __shared__ int s_flag;
 
   const int tile = tile_of_block(p.nblocks);
@@ -174,29 +221,27 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   if (threadIdx.x == 0) {
     s_flag = p.flag_in[tile];
     p.flag_in[tile] = 0;
+    s_cnt[0] = 0;
+    s_cnt[1] = 0;
   }
   __syncthreads();
   if (!s_flag) return;
+  if (threadIdx.x == 0) atomicAdd(p.tiles, 1ull);
 
-  TileCtx<G> c;
-  tile_setup<K, G, T>(p, tile, c);
+  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
+  const int x0 = tx * G::TW, y0 = ty * G::TH, z0 = tz * G::TD;
   const long long N = p.N;
+  // lattice extent of this tile (nodes past the lattice edge are never active)
+  const int lw = min(G::TW, p.W - x0), lh = min(G::TH, p.H - y0), ld = min(G::TD, p.D - z0);
 
-  int hj[K], tj[K];
-  long long gj[K];
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    hj[i] = (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i];
-    tj[i] = (p.dirs.dz[i] * G::TH + p.dirs.dy[i]) * G::TW + p.dirs.dx[i];
-    gj[i] = ((long long)p.dirs.dz[i] * p.H + p.dirs.dy[i]) * p.W + p.dirs.dx[i];
-  }
-
-  // stage the tile
+  // stage residuals and excess of the tile, labels with a halo
 #pragma unroll
   for (int j = 0; j < G::NPT; ++j) {
     const int t = threadIdx.x + j * NTHREADS;
-    const long long g = c.g[j];
-    if (g >= 0) {
+    int z, y, x;
+    tile_coords<G>(t, z, y, x);
+    if (z < ld && y < lh && x < lw) {
+      const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
 #pragma unroll
       for (int i = 0; i < K; ++i) s_res[i * NT + t] = p.res[i * N + g];
       s_ex[t] = p.ex[g];
@@ -207,60 +252,100 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
     }
     s_in[t] = 0;
   }
-  load_halo_h<G, T>(p, c, p.h, s_h);
+  for (int q = threadIdx.x; q < G::NH; q += NTHREADS) {
+    const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
+    const int gz = z0 + hz - G::RZ, gy = y0 + hy - G::RY, gx = x0 + hx - G::RX;
+    int v = INF_H;
+    if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W)
+      v = p.h[((long long)gz * p.H + gy) * p.W + gx];
+    s_h[q] = v;
+  }
 
   // collect flow that neighbouring tiles pushed into this tile
 #pragma unroll
   for (int j = 0; j < G::NPT; ++j) {
     const int t = threadIdx.x + j * NTHREADS;
-    const long long g = c.g[j];
-    const unsigned outside = c.m_lat[j] & ~c.m_tile[j];
-    if (g >= 0 && outside) {
+    int z, y, x;
+    tile_coords<G>(t, z, y, x);
+    const bool border = z < G::RZ || z >= G::TD - G::RZ || y < G::RY || y >= G::TH - G::RY ||
+                        x < G::RX || x >= G::TW - G::RX;
+    if (!border || z >= ld || y >= lh || x >= lw) continue;
+    const int gz = z0 + z, gy = y0 + y, gx = x0 + x;
+    const long long g = ((long long)gz * p.H + gy) * p.W + gx;
 #pragma unroll
-      for (int i = 0; i < K; ++i) {
-        if (outside & (1u << i)) {
-          T* slot = &p.mbox[(long long)((i + K / 2) % K) * N + g + gj[i]];
-          if (__builtin_nontemporal_load(slot) != 0) {
-            const T a = gtake(slot);
-            s_ex[t] += a;
-            s_res[i * NT + t] += a;
-          }
-        }
+    for (int i = 0; i < K; ++i) {
+      const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+      const int lz = z + dz, ly = y + dy, lx = x + dx;
+      if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) continue;
+      const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
+      if (nz < 0 || nz >= p.D || ny < 0 || ny >= p.H || nx < 0 || nx >= p.W) continue;
+      T* slot = &p.mbox[(long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx];
+      if (__builtin_nontemporal_load(slot) != 0) {
+        const T a = gtake(slot);
+        s_ex[t] += a;
+        s_res[i * NT + t] += a;
       }
     }
   }
   __syncthreads();
 
   for (int it = 0; it < p.iters; ++it) {
-    // push: labels frozen
+    // fold last iteration's inflow into the excess and list the active nodes
+    // (ex > 0, label < INF) with one LDS atomic per wave
+    int* cnt = &s_cnt[it & 1];
 #pragma unroll
     for (int j = 0; j < G::NPT; ++j) {
       const int t = threadIdx.x + j * NTHREADS;
-      const long long g = c.g[j];
-      if (g < 0) continue;
+      const T a = s_in[t];
       T e = s_ex[t];
-      const int hv = s_h[c.hi[j]];
-      if (e <= 0 || hv >= INF_H) continue;
+      if (a != 0) {
+        e += a;
+        s_ex[t] = e;
+        s_in[t] = 0;
+      }
+      int z, y, x;
+      tile_coords<G>(t, z, y, x);
+      const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
+      const bool act = e > 0 && s_h[hi] < INF_H;
+      const unsigned long long b = __ballot(act);
+      const int lane = threadIdx.x & 63;
+      int base = 0;
+      if (lane == 0 && b) base = atomicAdd(cnt, __popcll(b));
+      base = __shfl(base, 0, 64);
+      if (act) s_list[base + __popcll(b & ((1ull << lane) - 1))] = (unsigned short)t;
+    }
+    __syncthreads();
+    const int n = *cnt;
+    if (n == 0) break;
+    if (threadIdx.x == 0) s_cnt[(it + 1) & 1] = 0;
+
+    // push along downhill residual arcs, labels frozen
+    for (int q = threadIdx.x; q < n; q += NTHREADS) {
+      const int t = s_list[q];
+      int z, y, x;
+      tile_coords<G>(t, z, y, x);
+      const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
+      const int hv = s_h[hi];
+      T e = s_ex[t];
 #pragma unroll
       for (int i = 0; i < K; ++i) {
-        if (e > 0 && (c.m_lat[j] & (1u << i))) {
-          const int hn = s_h[c.hi[j] + hj[i]];
-          if (hn < hv) {
-            const T r = s_res[i * NT + t];
-            if (r > 0) {
-              const T d = r < e ? r : e;
-              s_res[i * NT + t] = r - d;
-              e -= d;
-              if (c.m_tile[j] & (1u << i)) {
-                const int tn = t + tj[i];
-                s_res[((i + K / 2) % K) * NT + tn] += d;
-                lds_add(&s_in[tn], d);
-              } else {
-                gadd(&p.mbox[(long long)i * N + g], d);
-                const int z = t / (G::TH * G::TW), y = (t / G::TW) % G::TH, x = t % G::TW;
-                flag_neighbor_tile<G, T>(p, c.z0 + z + p.dirs.dz[i], c.y0 + y + p.dirs.dy[i],
-                                         c.x0 + x + p.dirs.dx[i], p.flag_out);
-              }
+        const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+        const int hn = s_h[hi + (dz * G::HH + dy) * G::HW + dx];
+        if (e > 0 && hn < hv) {
+          const T r = s_res[i * NT + t];
+          if (r > 0) {
+            const T d = r < e ? r : e;
+            s_res[i * NT + t] = r - d;
+            e -= d;
+            const int lz = z + dz, ly = y + dy, lx = x + dx;
+            if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) {
+              const int tn = t + (dz * G::TH + dy) * G::TW + dx;
+              s_res[((i + K / 2) % K) * NT + tn] += d;
+              lds_add(&s_in[tn], d);
+            } else {
+              const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+              gadd(&p.mbox[(long long)i * N + g], d);
+              flag_neighbor_tile<G, T>(p, z0 + lz, y0 + ly, x0 + lx, p.flag_out);
             }
           }
         }
@@ -268,55 +353,42 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
       s_ex[t] = e;
     }
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < G::NPT; ++j) {
-      const int t = threadIdx.x + j * NTHREADS;
-      const T a = s_in[t];
-      if (a != 0) {
-        s_ex[t] += a;
-        s_in[t] = 0;
-      }
-    }
-    __syncthreads();
-    // relabel: residuals frozen; labels only grow, so concurrent reads are safe
-    bool act = false;
-#pragma unroll
-    for (int j = 0; j < G::NPT; ++j) {
-      const int t = threadIdx.x + j * NTHREADS;
-      if (c.g[j] < 0) continue;
-      const T e = s_ex[t];
-      const int hv = s_h[c.hi[j]];
-      if (e <= 0 || hv >= INF_H) continue;
+
+    // relabel listed nodes that kept excess, residuals frozen; labels only
+    // grow, so concurrent reads of a neighbour's label are safe
+    for (int q = threadIdx.x; q < n; q += NTHREADS) {
+      const int t = s_list[q];
+      if (s_ex[t] <= 0) continue;
+      int z, y, x;
+      tile_coords<G>(t, z, y, x);
+      const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
+      const int hv = s_h[hi];
       int m = INF_H;
 #pragma unroll
       for (int i = 0; i < K; ++i) {
-        if ((c.m_lat[j] & (1u << i)) && s_res[i * NT + t] > 0) {
-          const int hn = s_h[c.hi[j] + hj[i]];
+        if (s_res[i * NT + t] > 0) {
+          const int hn = s_h[hi + (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i]];
           m = hn < m ? hn : m;
         }
       }
-      if (m >= hv) {
-        const int nh = (m >= INF_H - 1) ? INF_H : m + 1;
-        s_h[c.hi[j]] = nh;
-        if (nh < INF_H) act = true;
-      } else {
-        act = true;
-      }
+      if (m >= hv) s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
     }
-    if (!__syncthreads_or(act)) break;
+    __syncthreads();
   }
-  __syncthreads();
 
+  // write the tile back (folding any inflow of the last iteration)
   bool act = false;
 #pragma unroll
   for (int j = 0; j < G::NPT; ++j) {
     const int t = threadIdx.x + j * NTHREADS;
-    const long long g = c.g[j];
-    if (g < 0) continue;
+    int z, y, x;
+    tile_coords<G>(t, z, y, x);
+    if (z >= ld || y >= lh || x >= lw) continue;
+    const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
 #pragma unroll
     for (int i = 0; i < K; ++i) p.res[i * N + g] = s_res[i * NT + t];
-    const T e = s_ex[t];
-    const int hv = s_h[c.hi[j]];
+    const T e = s_ex[t] + s_in[t];
+    const int hv = s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)];
     p.ex[g] = e;
     p.h[g] = hv;
     if (e > 0 && hv < INF_H) act = true;
@@ -372,92 +444,188 @@ __global__ __launch_bounds__(NTHREADS) void k_bfs_init(KP<T> p, MT* __restrict__
   p.h[v] = p.ex[v] < 0 ? 0 : INF_H;
 }
 
-template <int K, class G, typename T, typename MT>
-__global__ __launch_bounds__(NTHREADS) void k_bfs_tile(KP<T> p, const MT* __restrict__ mask) {
-  __shared__ int s_d[G::NH];
+// Tiles of the exact passes (BFS labels, S_min flood) are larger than the push
+// tiles: only labels / reach bits and 1-bit arc masks are staged in LDS.
+constexpr int BTHREADS = 512;
+
+template <int TD_, int TH_, int TW_, int RZ_, int RY_, int RX_>
+struct BGeo {
+  static constexpr int TD = TD_, TH = TH_, TW = TW_, RZ = RZ_, RY = RY_, RX = RX_;
+  static constexpr int NT = TD * TH * TW;
+  static constexpr int HD = TD + 2 * RZ, HH = TH + 2 * RY, HW = TW + 2 * RX;
+  static constexpr int NH = HD * HH * HW;
+  static constexpr int NPT = NT / BTHREADS;
+  static_assert(NT % BTHREADS == 0, "tile must be a multiple of the block");
+};
+
+struct BP {
+  int D, H, W;
+  long long N;
+  int ntx, nty, ntz, ntiles, nblocks;
+  int* flag_in;
+  int* flag_out;
+  unsigned long long* counter;
+  DirTab dirs;
+};
+
+template <class B>
+__device__ inline int bhalo(int t) {
+  const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
+  return ((z + B::RZ) * B::HH + (y + B::RY)) * B::HW + (x + B::RX);
+}
+
+template <class B>
+__device__ inline bool bborder(int t) {
+  const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
+  return z < B::RZ || z >= B::TD - B::RZ || y < B::RY || y >= B::TH - B::RY || x < B::RX || x >= B::TW - B::RX;
+}
+
+// Flag the 3^d - 1 neighbouring tiles and count a border change.
+__device__ inline void flag_around(const BP& p, int tile) {
+  atomicAdd(p.counter, 1ull);
+  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
+  for (int a = -1; a <= 1; ++a)
+    for (int b = -1; b <= 1; ++b)
+      for (int e = -1; e <= 1; ++e) {
+        const int z = tz + a, y = ty + b, x = tx + e;
+        if ((a | b | e) == 0 || z < 0 || z >= p.ntz || y < 0 || y >= p.nty || x < 0 || x >= p.ntx) continue;
+        p.flag_out[(z * p.nty + y) * p.ntx + x] = 1;
+      }
+}
+
+// One tile of the exact distance labelling d(v) = 1 + min{d(w) : arc v->w
+// residual}, d = 0 on sink-capable nodes.  The tile first pulls every node once
+// from its neighbours (halo included), then only nodes whose label dropped
+// relax their in-neighbours inside the tile, until no label changes.  Labels
+// only decrease and every value is the length of a real residual path, so the
+// fixpoint over all tiles is the exact BFS distance.
+template <int K, class B, typename MT>
+__global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restrict__ mask, int* __restrict__ hgl) {
+  __shared__ int s_d[B::NH];
+  __shared__ MT s_m[B::NT];
+  __shared__ unsigned short s_f[2][B::NT];
+  __shared__ __align__(4) unsigned char s_q[B::NT];
+  __shared__ int s_n[2];
   __shared__ int s_flag;
   const int tile = tile_of_block(p.nblocks);
   if (tile >= p.ntiles) return;
   if (threadIdx.x == 0) {
     s_flag = p.flag_in[tile];
     p.flag_in[tile] = 0;
+    s_n[0] = 0;
+    s_n[1] = 0;
   }
   __syncthreads();
   if (!s_flag) return;
-  TileCtx<G> c;
-  tile_setup<K, G, T>(p, tile, c);
-  int hj[K];
+  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
+  const int x0 = tx * B::TW, y0 = ty * B::TH, z0 = tz * B::TD;
+  const int lw = min(B::TW, p.W - x0), lh = min(B::TH, p.H - y0), ld = min(B::TD, p.D - z0);
+  for (int q = threadIdx.x; q < B::NH; q += BTHREADS) {
+    const int hz = q / (B::HH * B::HW), hy = (q / B::HW) % B::HH, hx = q % B::HW;
+    const int gz = z0 + hz - B::RZ, gy = y0 + hy - B::RY, gx = x0 + hx - B::RX;
+    int v = INF_H;
+    if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W)
+      v = hgl[((long long)gz * p.H + gy) * p.W + gx];
+    s_d[q] = v;
+  }
+  int d0[B::NPT];
 #pragma unroll
-  for (int i = 0; i < K; ++i) hj[i] = (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i];
-  MT mk[G::NPT];
-  int d0[G::NPT];
-#pragma unroll
-  for (int j = 0; j < G::NPT; ++j) mk[j] = c.g[j] >= 0 ? mask[c.g[j]] : (MT)0;
-  load_halo_h<G, T>(p, c, p.h, s_d);
+  for (int j = 0; j < B::NPT; ++j) {
+    const int t = threadIdx.x + j * BTHREADS;
+    const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
+    MT m = 0;
+    if (z < ld && y < lh && x < lw) m = mask[((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x)];
+    s_m[t] = m;
+    s_q[t] = 0;
+  }
   __syncthreads();
+  // pull pass
 #pragma unroll
-  for (int j = 0; j < G::NPT; ++j) d0[j] = s_d[c.hi[j]];
+  for (int j = 0; j < B::NPT; ++j) {
+    const int t = threadIdx.x + j * BTHREADS;
+    const int hi = bhalo<B>(t);
+    const int dv = s_d[hi];
+    d0[j] = dv;
+    const MT m = s_m[t];
+    if (dv == 0 || m == 0) continue;
+    int best = dv;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+      if (m & ((MT)1 << i)) {
+        const int dn = s_d[hi + (p.dirs.dz[i] * B::HH + p.dirs.dy[i]) * B::HW + p.dirs.dx[i]] + 1;
+        best = dn < best ? dn : best;
+      }
+    if (best < dv) {
+      atomicMin(&s_d[hi], best);
+      s_q[t] = 1;
+      s_f[0][atomicAdd(&s_n[0], 1)] = (unsigned short)t;
+    }
+  }
+  __syncthreads();
+  int cur = 0;
   for (;;) {
-    bool ch = false;
+    const int n = s_n[cur];
+    if (n == 0) break;
+    for (int q = threadIdx.x; q < n; q += BTHREADS) {
+      const int v = s_f[cur][q];
+      // release v's list flag before reading its label: a later drop of the
+      // label then re-lists v instead of being lost
+      atomicAnd((unsigned int*)&s_q[v & ~3], ~(0xffu << ((v & 3) * 8)));
+      __threadfence_block();
+      const int hv = bhalo<B>(v);
+      const int nd = s_d[hv] + 1;
+      const int z = v / (B::TH * B::TW), y = (v / B::TW) % B::TH, x = v % B::TW;
 #pragma unroll
-    for (int j = 0; j < G::NPT; ++j) {
-      if (c.g[j] < 0) continue;
-      const int dv = s_d[c.hi[j]];
-      if (dv == 0) continue;
-      int best = dv;
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        if (mk[j] & ((MT)1 << i)) {
-          const int dn = s_d[c.hi[j] + hj[i]] + 1;
-          best = dn < best ? dn : best;
+      for (int j = 0; j < K; ++j) {
+        const int dz = p.dirs.dz[j], dy = p.dirs.dy[j], dx = p.dirs.dx[j];
+        const int uz = z + dz, uy = y + dy, ux = x + dx;
+        if (uz < 0 || uz >= B::TD || uy < 0 || uy >= B::TH || ux < 0 || ux >= B::TW) continue;
+        const int u = v + (dz * B::TH + dy) * B::TW + dx;
+        // arc u -> v is u's direction rev(j)
+        if (!(s_m[u] & ((MT)1 << ((j + K / 2) % K)))) continue;
+        const int hu = hv + (dz * B::HH + dy) * B::HW + dx;
+        if (s_d[hu] > nd && atomicMin(&s_d[hu], nd) > nd) {
+          // claim u for the next list (one byte flag, set through its word)
+          unsigned int* wd = (unsigned int*)&s_q[u & ~3];
+          const int sh = (u & 3) * 8;
+          const unsigned int old = atomicOr(wd, 1u << sh);
+          if (!((old >> sh) & 0xffu)) s_f[cur ^ 1][atomicAdd(&s_n[cur ^ 1], 1)] = (unsigned short)u;
         }
       }
-      if (best < dv) {
-        s_d[c.hi[j]] = best;
-        ch = true;
-      }
     }
-    if (!__syncthreads_or(ch)) break;
+    __syncthreads();
+    if (threadIdx.x == 0) s_n[cur] = 0;
+    cur ^= 1;
+    __syncthreads();
   }
   bool border = false;
 #pragma unroll
-  for (int j = 0; j < G::NPT; ++j) {
-    if (c.g[j] < 0) continue;
-    const int dv = s_d[c.hi[j]];
+  for (int j = 0; j < B::NPT; ++j) {
+    const int t = threadIdx.x + j * BTHREADS;
+    const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
+    if (z >= ld || y >= lh || x >= lw) continue;
+    const int dv = s_d[bhalo<B>(t)];
     if (dv != d0[j]) {
-      p.h[c.g[j]] = dv;
-      if (c.m_lat[j] & ~c.m_tile[j]) border = true;
+      hgl[((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x)] = dv;
+      if (bborder<B>(t)) border = true;
     }
   }
-  if (__syncthreads_or(border)) {
-    if (threadIdx.x == 0) {
-      atomicAdd(p.counter, 1ull);
-      const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
-      for (int a = -1; a <= 1; ++a)
-        for (int b = -1; b <= 1; ++b)
-          for (int e = -1; e <= 1; ++e) {
-            const int z = tz + a, y = ty + b, x = tx + e;
-            if ((a | b | e) == 0 || z < 0 || z >= p.ntz || y < 0 || y >= p.nty || x < 0 || x >= p.ntx) continue;
-            p.flag_out[(z * p.nty + y) * p.ntx + x] = 1;
-          }
-    }
-  }
+  if (__syncthreads_or(border) && threadIdx.x == 0) flag_around(p, tile);
 }
 
 // count active nodes (ex > 0, h < INF) and flag their tiles for the push rounds
 template <class G, typename T>
-__global__ __launch_bounds__(NTHREADS) void k_count_active(KP<T> p) {
-  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
-  bool a = false;
-  if (v < p.N) {
-    a = p.ex[v] > 0 && p.h[v] < INF_H;
-    if (a) {
+__global__ __launch_bounds__(NTHREADS) void k_count_active(KP<T> p, long long* __restrict__ part) {
+  long long cnt = 0;
+  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < p.N; v += (long long)gridDim.x * NTHREADS) {
+    if (p.ex[v] > 0 && p.h[v] < INF_H) {
+      ++cnt;
       const int x = (int)(v % p.W), y = (int)((v / p.W) % p.H), z = (int)(v / ((long long)p.W * p.H));
       flag_neighbor_tile<G, T>(p, z, y, x, p.flag_out);
     }
   }
-  const unsigned long long b = __ballot(a);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(p.counter, (unsigned long long)__popcll(b));
+  cnt = block_reduce<false>(cnt);
+  if (threadIdx.x == 0) part[blockIdx.x] = cnt;
 }
 
 // ---- S_min: flood fill from excess nodes over residual arcs --------------------
@@ -469,83 +637,110 @@ __global__ __launch_bounds__(NTHREADS) void k_flood_init(KP<T> p, int* __restric
   reach[v] = p.ex[v] > 0 ? 1 : 0;
 }
 
-template <int K, class G, typename T, typename MT>
-__global__ __launch_bounds__(NTHREADS) void k_flood_tile(KP<T> p, const MT* __restrict__ mask, int* __restrict__ reach) {
-  __shared__ int s_r[G::NH];
-  __shared__ MT s_m[G::NH];
+template <int K, class B, typename MT>
+__global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restrict__ mask, int* __restrict__ reach) {
+  __shared__ __align__(4) unsigned char s_r[(B::NH + 3) & ~3];
+  __shared__ MT s_m[B::NH];
+  __shared__ unsigned short s_f[2][B::NT];
+  __shared__ int s_n[2];
   __shared__ int s_flag;
   const int tile = tile_of_block(p.nblocks);
   if (tile >= p.ntiles) return;
   if (threadIdx.x == 0) {
     s_flag = p.flag_in[tile];
     p.flag_in[tile] = 0;
+    s_n[0] = 0;
+    s_n[1] = 0;
   }
   __syncthreads();
   if (!s_flag) return;
-  TileCtx<G> c;
-  tile_setup<K, G, T>(p, tile, c);
-  int hj[K];
-#pragma unroll
-  for (int i = 0; i < K; ++i) hj[i] = (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i];
-  for (int q = threadIdx.x; q < G::NH; q += NTHREADS) {
-    const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
-    const int gz = c.z0 + hz - G::RZ, gy = c.y0 + hy - G::RY, gx = c.x0 + hx - G::RX;
-    int r = 0;
+  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
+  const int x0 = tx * B::TW, y0 = ty * B::TH, z0 = tz * B::TD;
+  const int lw = min(B::TW, p.W - x0), lh = min(B::TH, p.H - y0), ld = min(B::TD, p.D - z0);
+  for (int q = threadIdx.x; q < B::NH; q += BTHREADS) {
+    const int hz = q / (B::HH * B::HW), hy = (q / B::HW) % B::HH, hx = q % B::HW;
+    const int gz = z0 + hz - B::RZ, gy = y0 + hy - B::RY, gx = x0 + hx - B::RX;
+    unsigned char r = 0;
     MT m = 0;
     if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W) {
       const long long g = ((long long)gz * p.H + gy) * p.W + gx;
-      r = reach[g];
+      r = (unsigned char)reach[g];
       m = mask[g];
     }
     s_r[q] = r;
     s_m[q] = m;
   }
   __syncthreads();
-  int r0[G::NPT];
+  int r0[B::NPT];
+  // pull pass: w becomes reached when an in-neighbour v = w + dir[i] is reached
+  // and v's arc rev(i) (pointing at w) is residual
 #pragma unroll
-  for (int j = 0; j < G::NPT; ++j) r0[j] = s_r[c.hi[j]];
-  for (;;) {
-    bool ch = false;
+  for (int j = 0; j < B::NPT; ++j) {
+    const int t = threadIdx.x + j * BTHREADS;
+    const int hi = bhalo<B>(t);
+    r0[j] = s_r[hi];
+    const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
+    if (r0[j] || z >= ld || y >= lh || x >= lw) continue;
 #pragma unroll
-    for (int j = 0; j < G::NPT; ++j) {
-      if (c.g[j] < 0 || s_r[c.hi[j]]) continue;
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        // in-arc from v = w + dir[i]: v's arc along rev(i) points at w
-        if (c.m_lat[j] & (1u << i)) {
-          const int q = c.hi[j] + hj[i];
-          if (s_r[q] && (s_m[q] & ((MT)1 << ((i + K / 2) % K)))) {
-            s_r[c.hi[j]] = 1;
-            ch = true;
-            break;
-          }
-        }
+    for (int i = 0; i < K; ++i) {
+      const int q = hi + (p.dirs.dz[i] * B::HH + p.dirs.dy[i]) * B::HW + p.dirs.dx[i];
+      if (s_r[q] && (s_m[q] & ((MT)1 << ((i + K / 2) % K)))) {
+        s_r[hi] = 1;
+        s_f[0][atomicAdd(&s_n[0], 1)] = (unsigned short)t;
+        break;
       }
     }
-    if (!__syncthreads_or(ch)) break;
+  }
+  __syncthreads();
+  // every owned node that was reached at the start also spreads inside the tile
+#pragma unroll
+  for (int j = 0; j < B::NPT; ++j) {
+    const int t = threadIdx.x + j * BTHREADS;
+    if (r0[j]) s_f[0][atomicAdd(&s_n[0], 1)] = (unsigned short)t;
+  }
+  __syncthreads();
+  int cur = 0;
+  for (;;) {
+    const int n = s_n[cur];
+    if (n == 0) break;
+    for (int q = threadIdx.x; q < n; q += BTHREADS) {
+      const int v = s_f[cur][q];
+      const int hv = bhalo<B>(v);
+      const MT m = s_m[hv];
+      const int z = v / (B::TH * B::TW), y = (v / B::TW) % B::TH, x = v % B::TW;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        if (!(m & ((MT)1 << i))) continue;
+        const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+        const int wz = z + dz, wy = y + dy, wx = x + dx;
+        if (wz < 0 || wz >= B::TD || wy < 0 || wy >= B::TH || wx < 0 || wx >= B::TW) continue;
+        const int hw = hv + (dz * B::HH + dy) * B::HW + dx;
+        if (s_r[hw]) continue;
+        // claim w through the byte's containing word
+        unsigned int* wd = (unsigned int*)&s_r[hw & ~3];
+        const int sh = (hw & 3) * 8;
+        const unsigned int old = atomicOr(wd, 1u << sh);
+        if (!((old >> sh) & 0xffu))
+          s_f[cur ^ 1][atomicAdd(&s_n[cur ^ 1], 1)] = (unsigned short)(v + (dz * B::TH + dy) * B::TW + dx);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_n[cur] = 0;
+    cur ^= 1;
+    __syncthreads();
   }
   bool border = false;
 #pragma unroll
-  for (int j = 0; j < G::NPT; ++j) {
-    if (c.g[j] < 0) continue;
-    if (s_r[c.hi[j]] != r0[j]) {
-      reach[c.g[j]] = 1;
-      if (c.m_lat[j] & ~c.m_tile[j]) border = true;
+  for (int j = 0; j < B::NPT; ++j) {
+    const int t = threadIdx.x + j * BTHREADS;
+    const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
+    if (z >= ld || y >= lh || x >= lw) continue;
+    if (s_r[bhalo<B>(t)] && !r0[j]) {
+      reach[((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x)] = 1;
+      if (bborder<B>(t)) border = true;
     }
   }
-  if (__syncthreads_or(border)) {
-    if (threadIdx.x == 0) {
-      atomicAdd(p.counter, 1ull);
-      const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
-      for (int a = -1; a <= 1; ++a)
-        for (int b = -1; b <= 1; ++b)
-          for (int e = -1; e <= 1; ++e) {
-            const int z = tz + a, y = ty + b, x = tx + e;
-            if ((a | b | e) == 0 || z < 0 || z >= p.ntz || y < 0 || y >= p.nty || x < 0 || x >= p.ntx) continue;
-            p.flag_out[(z * p.nty + y) * p.ntx + x] = 1;
-          }
-    }
-  }
+  if (__syncthreads_or(border) && threadIdx.x == 0) flag_around(p, tile);
 }
 
 // ---- outputs -------------------------------------------------------------------
@@ -553,16 +748,15 @@ __global__ __launch_bounds__(NTHREADS) void k_flood_tile(KP<T> p, const MT* __re
 // flow = sum(max(red0,0)) - sum(max(-ex,0)), exact in int64
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void k_flow(KP<T> p, const long long* __restrict__ red0,
-                                                   unsigned long long* out) {
-  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+                                                   long long* __restrict__ part) {
   long long f = 0;
-  if (v < p.N) {
+  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < p.N; v += (long long)gridDim.x * NTHREADS) {
     const long long r = red0[v];
     const long long e = (long long)p.ex[v];
-    f = (r > 0 ? r : 0) - (e < 0 ? -e : 0);
+    f += (r > 0 ? r : 0) - (e < 0 ? -e : 0);
   }
-  for (int o = 32; o > 0; o >>= 1) f += __shfl_down(f, o, 64);
-  if ((threadIdx.x & 63) == 0 && f != 0) atomicAdd(out, (unsigned long long)f);
+  f = block_reduce<false>(f);
+  if (threadIdx.x == 0) part[blockIdx.x] = f;
 }
 
 template <typename T>
@@ -623,18 +817,17 @@ __global__ __launch_bounds__(NTHREADS) void k_reparam(int D, int H, int W, long 
 }
 
 // working state from the base network.  err[0] |= 2 on a negative arc
-// capacity; mx[0] / mx[1] receive max |red0| and max alpha0, which bound every
-// excess (|red0| + k * max alpha) and residual (2 * max alpha) of the solve.
+// capacity; part[2b], part[2b+1] receive block maxima of |red0| and alpha0,
+// which bound every excess (|red0| + k max alpha) and residual (2 max alpha).
 template <int K, typename T>
 __global__ __launch_bounds__(NTHREADS) void k_load(KP<T> p, const long long* __restrict__ red0,
                                                    const long long* __restrict__ alpha0, int* err,
-                                                   unsigned long long* mx) {
-  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+                                                   long long* __restrict__ part) {
   long long mr = 0, ma = 0;
   bool bad = false;
-  if (v < p.N) {
+  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < p.N; v += (long long)gridDim.x * NTHREADS) {
     const long long r = red0[v];
-    mr = r < 0 ? -r : r;
+    mr = (r < 0 ? -r : r) > mr ? (r < 0 ? -r : r) : mr;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       const long long a = alpha0[(long long)i * p.N + v];
@@ -645,14 +838,11 @@ __global__ __launch_bounds__(NTHREADS) void k_load(KP<T> p, const long long* __r
     }
     p.ex[v] = (T)(-r);
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    const long long a = __shfl_down(mr, o, 64), b = __shfl_down(ma, o, 64);
-    mr = a > mr ? a : mr;
-    ma = b > ma ? b : ma;
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(&mx[0], (unsigned long long)mr);
-    atomicMax(&mx[1], (unsigned long long)ma);
+  mr = block_reduce<true>(mr);
+  ma = block_reduce<true>(ma);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = mr;
+    part[2 * blockIdx.x + 1] = ma;
   }
   if (bad) atomicOr(err, 2);
 }
@@ -667,7 +857,7 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + NTHREADS - 1) /
 
 // ---- engine -----------------------------------------------------------------------
 
-template <int K, class G, typename T, typename MT>
+template <int K, class G, class B, typename T, typename MT>
 class Engine final : public MinCutEngine {
  public:
   Engine(const LatticeGeom& geo) : g_(geo) {
@@ -681,11 +871,19 @@ class Engine final : public MinCutEngine {
     if (ntiles_ > (1LL << 30)) fail(LF_ERR_BAD_ARG, "lattice too large");
     p_.ntiles = (int)ntiles_;
     p_.nblocks = (int)(((ntiles_ + 7) / 8) * 8);
+    bp_.D = p_.D; bp_.H = p_.H; bp_.W = p_.W; bp_.N = N_;
+    bp_.ntx = (int)((geo.W + B::TW - 1) / B::TW);
+    bp_.nty = (int)((geo.H + B::TH - 1) / B::TH);
+    bp_.ntz = (int)((geo.D + B::TD - 1) / B::TD);
+    bntiles_ = (long long)bp_.ntx * bp_.nty * bp_.ntz;
+    bp_.ntiles = (int)bntiles_;
+    bp_.nblocks = (int)(((bntiles_ + 7) / 8) * 8);
     for (int i = 0; i < K; ++i) {
       p_.dirs.dz[i] = (int8_t)geo.dz[i];
       p_.dirs.dy[i] = (int8_t)geo.dy[i];
       p_.dirs.dx[i] = (int8_t)geo.dx[i];
     }
+    bp_.dirs = p_.dirs;
     LF_HIP(hipMalloc(&red0_, sizeof(long long) * N_));
     LF_HIP(hipMalloc(&alpha0_, sizeof(long long) * N_ * K));
     LF_HIP(hipMalloc(&res_, sizeof(T) * N_ * K));
@@ -696,15 +894,22 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipMalloc(&reach_, sizeof(int) * N_));
     LF_HIP(hipMalloc(&flags_[0], sizeof(int) * ntiles_));
     LF_HIP(hipMalloc(&flags_[1], sizeof(int) * ntiles_));
+    LF_HIP(hipMalloc(&bflags_[0], sizeof(int) * bntiles_));
+    LF_HIP(hipMalloc(&bflags_[1], sizeof(int) * bntiles_));
     LF_HIP(hipMalloc(&dev_small_, 64));
+    LF_HIP(hipMalloc(&part_, sizeof(long long) * 2 * RED_BLOCKS));
     LF_HIP(hipHostMalloc(&host_small_, 64, hipHostMallocDefault));
+    LF_HIP(hipEventCreate(&ev_[0]));
+    LF_HIP(hipEventCreate(&ev_[1]));
     reset_energies();
   }
   ~Engine() override {
     hipStreamSynchronize(s_);
     hipFree(red0_); hipFree(alpha0_); hipFree(res_); hipFree(mbox_); hipFree(ex_); hipFree(h_);
-    hipFree(mask_); hipFree(reach_); hipFree(flags_[0]); hipFree(flags_[1]); hipFree(dev_small_);
+    hipFree(mask_); hipFree(reach_); hipFree(flags_[0]); hipFree(flags_[1]); hipFree(bflags_[0]); hipFree(bflags_[1]); hipFree(dev_small_); hipFree(part_);
     hipHostFree(host_small_);
+    hipEventDestroy(ev_[0]);
+    hipEventDestroy(ev_[1]);
     hipStreamDestroy(s_);
   }
 
@@ -742,12 +947,15 @@ class Engine final : public MinCutEngine {
     const auto t0 = clk::now();
     stats = lf_stats{};
     KP<T> p = params();
-    p.iters = tun.local_iters > 0 ? tun.local_iters : 8;
-    const int R = tun.rounds_per_relabel > 0 ? tun.rounds_per_relabel : 16;
+    p.iters = tun.local_iters > 0 ? tun.local_iters : 4;
+    const int R = tun.rounds_per_relabel > 0 ? tun.rounds_per_relabel : 4;
     int* err = reinterpret_cast<int*>(dev_small_);
     unsigned long long* mx = reinterpret_cast<unsigned long long*>(dev_small_ + 32);
     LF_HIP(hipMemsetAsync(dev_small_, 0, 64, s_));
-    hipLaunchKernelGGL((k_load<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, red0_, alpha0_, err, mx);
+    const int lb = red_blocks();
+    hipLaunchKernelGGL((k_load<K, T>), dim3(lb), dim3(NTHREADS), 0, s_, p, red0_, alpha0_, err, part_);
+    hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 0, mx);
+    hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 1, mx + 1);
     LF_HIP(hipGetLastError());
     LF_HIP(hipMemcpyAsync(host_small_, dev_small_, 64, hipMemcpyDeviceToHost, s_));
     LF_HIP(hipStreamSynchronize(s_));
@@ -762,6 +970,7 @@ class Engine final : public MinCutEngine {
                                              : "capacities exceed the int64 range");
     }
     double t_push = 0, t_bfs = 0;
+    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, tiles_ptr(), 0ull, 1LL);
     bool pushed = false;
     for (;;) {
       const auto tb = clk::now();
@@ -771,7 +980,7 @@ class Engine final : public MinCutEngine {
       if (active == 0) break;
       if (tun.max_cycles > 0 && stats.cycles > tun.max_cycles)
         fail(LF_ERR_STATE, "max_cycles reached before the flow became maximum");
-      const auto tp = clk::now();
+      LF_HIP(hipEventRecord(ev_[0], s_));
       for (int r = 0; r < R; ++r) {
         KP<T> q = p;
         q.flag_in = flags_[cur_];
@@ -781,31 +990,32 @@ class Engine final : public MinCutEngine {
         ++stats.rounds;
       }
       LF_HIP(hipGetLastError());
+      LF_HIP(hipEventRecord(ev_[1], s_));
       pushed = true;
-      LF_HIP(hipStreamSynchronize(s_));
-      t_push += std::chrono::duration<double>(clk::now() - tp).count();
+      LF_HIP(hipEventSynchronize(ev_[1]));
+      float ms = 0;
+      LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+      t_push += ms * 1e-3;
     }
+    const auto tf = clk::now();
     // S_min flood
     hipLaunchKernelGGL((k_flood_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, reach_);
-    fill_flags(flags_[cur_], 1);
-    fill_flags(flags_[cur_ ^ 1], 0);
-    for (;;) {
-      zero_counter();
-      KP<T> q = p;
-      q.flag_in = flags_[cur_];
-      q.flag_out = flags_[cur_ ^ 1];
-      hipLaunchKernelGGL((k_flood_tile<K, G, T, MT>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q, mask_, reach_);
-      cur_ ^= 1;
+    run_tiled([&](const BP& q) {
+      hipLaunchKernelGGL((k_flood_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, reach_);
       ++stats.flood_launches;
-      if (read_counter() == 0) break;
-    }
+    });
+    stats.flood_seconds = std::chrono::duration<double>(clk::now() - tf).count();
     // flow value
-    zero_counter();
-    hipLaunchKernelGGL((k_flow<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, red0_, counter_ptr());
+    hipLaunchKernelGGL((k_flow<T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, p, red0_, part_);
+    hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
     flow = (long long)read_counter();
     LF_HIP(hipStreamSynchronize(s_));
     stats.push_seconds = t_push;
     stats.bfs_seconds = t_bfs;
+    stats.tile_nodes = G::NT;
+    stats.bytes_per_node = 2 * ((long long)K * sizeof(T) + sizeof(T) + 4);
+    LF_HIP(hipMemcpy(host_small_, tiles_ptr(), sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    stats.tiles_pushed = (long long)*reinterpret_cast<unsigned long long*>(host_small_);
     stats.seconds = std::chrono::duration<double>(clk::now() - t0).count();
     solved = true;
   }
@@ -828,11 +1038,14 @@ class Engine final : public MinCutEngine {
     KP<T> p = p_;
     p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_;
     p.counter = counter_ptr();
+    p.tiles = tiles_ptr();
     p.flag_in = flags_[cur_];
     p.flag_out = flags_[cur_ ^ 1];
     return p;
   }
   unsigned long long* counter_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 16); }
+  int red_blocks() const { return (int)std::min<long long>(RED_BLOCKS, std::max<long long>(1, blocks_for(N_))); }
+  unsigned long long* tiles_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 24); }
   void zero_counter() {
     hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, counter_ptr(), 0ull, 1LL);
   }
@@ -841,8 +1054,31 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipStreamSynchronize(s_));
     return *reinterpret_cast<unsigned long long*>(host_small_);
   }
-  void fill_flags(int* f, int v) {
-    hipLaunchKernelGGL((k_fill<int>), dim3(blocks_for(ntiles_)), dim3(NTHREADS), 0, s_, f, v, ntiles_);
+  void fill_flags(int* f, int v, long long n = -1) {
+    if (n < 0) n = ntiles_;
+    hipLaunchKernelGGL((k_fill<int>), dim3(blocks_for(n)), dim3(NTHREADS), 0, s_, f, v, n);
+  }
+
+  // Repeat a tiled exact pass until no tile border changes.  Launches go in
+  // batches; only the last launch of a batch is checked, launches over
+  // converged tiles exit at their flag test.
+  template <class F>
+  void run_tiled(F&& launch) {
+    fill_flags(bflags_[bcur_], 1, bntiles_);
+    fill_flags(bflags_[bcur_ ^ 1], 0, bntiles_);
+    for (int batch = 2;; batch = std::min(batch * 2, 8)) {
+      for (int b = 0; b < batch; ++b) {
+        if (b == batch - 1) zero_counter();
+        BP q = bp_;
+        q.flag_in = bflags_[bcur_];
+        q.flag_out = bflags_[bcur_ ^ 1];
+        q.counter = counter_ptr();
+        launch(q);
+        bcur_ ^= 1;
+      }
+      LF_HIP(hipGetLastError());
+      if (read_counter() == 0) break;
+    }
   }
 
   // deliver mailboxes, exact BFS labels, flag active tiles; returns #active nodes
@@ -850,25 +1086,17 @@ class Engine final : public MinCutEngine {
     if (pushed)
       hipLaunchKernelGGL((k_deliver<K, G, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
     hipLaunchKernelGGL((k_bfs_init<K, T, MT>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, mask_);
-    fill_flags(flags_[cur_], 1);
-    fill_flags(flags_[cur_ ^ 1], 0);
-    for (;;) {
-      zero_counter();
-      KP<T> q = p;
-      q.flag_in = flags_[cur_];
-      q.flag_out = flags_[cur_ ^ 1];
-      hipLaunchKernelGGL((k_bfs_tile<K, G, T, MT>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q, mask_);
-      cur_ ^= 1;
+    run_tiled([&](const BP& q) {
+      hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, h_);
       ++stats.bfs_launches;
-      if (read_counter() == 0) break;
-    }
+    });
     // push rounds start with exactly the tiles holding active nodes
     fill_flags(flags_[cur_], 0);
     fill_flags(flags_[cur_ ^ 1], 0);
-    zero_counter();
     KP<T> q = p;
     q.flag_out = flags_[cur_];
-    hipLaunchKernelGGL((k_count_active<G, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, q);
+    hipLaunchKernelGGL((k_count_active<G, T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, q, part_);
+    hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
     LF_HIP(hipGetLastError());
     return (long long)read_counter();
   }
@@ -887,16 +1115,22 @@ class Engine final : public MinCutEngine {
   int* reach_ = nullptr;
   int* flags_[2] = {nullptr, nullptr};
   int cur_ = 0;
+  BP bp_{};
+  long long bntiles_ = 0;
+  int* bflags_[2] = {nullptr, nullptr};
+  int bcur_ = 0;
   char* dev_small_ = nullptr;
+  long long* part_ = nullptr;
   void* host_small_ = nullptr;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
 };
 
-template <int K, class G>
+template <int K, class G, class B>
 std::unique_ptr<MinCutEngine> make_k(const LatticeGeom& g, int cap_bytes) {
   using MT = typename std::conditional<(K <= 8), unsigned char,
              typename std::conditional<(K <= 16), unsigned short, unsigned int>::type>::type;
-  if (cap_bytes == 4) return std::unique_ptr<MinCutEngine>(new Engine<K, G, int, MT>(g));
-  return std::unique_ptr<MinCutEngine>(new Engine<K, G, long long, MT>(g));
+  if (cap_bytes == 4) return std::unique_ptr<MinCutEngine>(new Engine<K, G, B, int, MT>(g));
+  return std::unique_ptr<MinCutEngine>(new Engine<K, G, B, long long, MT>(g));
 }
 
 int max_abs(const std::vector<int>& v) {
@@ -913,19 +1147,22 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
   const bool is3d = g.ndim == 3;
   if (!is3d && rz == 0 && ry <= 1 && rx <= 1) {
     using G = Geo<1, 16, 64, 0, 1, 1>;
-    if (g.k == 2) return make_k<2, G>(g, cap_bytes);
-    if (g.k == 4) return make_k<4, G>(g, cap_bytes);
-    if (g.k == 8) return make_k<8, G>(g, cap_bytes);
+    using B = BGeo<1, 64, 64, 0, 1, 1>;
+    if (g.k == 2) return make_k<2, G, B>(g, cap_bytes);
+    if (g.k == 4) return make_k<4, G, B>(g, cap_bytes);
+    if (g.k == 8) return make_k<8, G, B>(g, cap_bytes);
   }
   if (!is3d && rz == 0 && ry <= 2 && rx <= 2) {
     using G = Geo<1, 16, 64, 0, 2, 2>;
-    if (g.k == 12) return make_k<12, G>(g, cap_bytes);
-    if (g.k == 16) return make_k<16, G>(g, cap_bytes);
+    using B = BGeo<1, 64, 64, 0, 2, 2>;
+    if (g.k == 12) return make_k<12, G, B>(g, cap_bytes);
+    if (g.k == 16) return make_k<16, G, B>(g, cap_bytes);
   }
   if (is3d && rz <= 1 && ry <= 1 && rx <= 1) {
-    if (g.k == 6) return make_k<6, Geo<4, 8, 32, 1, 1, 1>>(g, cap_bytes);
-    if (g.k == 18) return make_k<18, Geo<4, 4, 16, 1, 1, 1>>(g, cap_bytes);
-    if (g.k == 26) return make_k<26, Geo<4, 4, 16, 1, 1, 1>>(g, cap_bytes);
+    using B = BGeo<16, 16, 16, 1, 1, 1>;
+    if (g.k == 6) return make_k<6, Geo<4, 8, 32, 1, 1, 1>, B>(g, cap_bytes);
+    if (g.k == 18) return make_k<18, Geo<4, 4, 16, 1, 1, 1>, B>(g, cap_bytes);
+    if (g.k == 26) return make_k<26, Geo<4, 4, 16, 1, 1, 1>, B>(g, cap_bytes);
   }
   fail(LF_ERR_UNSUPPORTED, "no device path for this kernel size / radius yet");
 }
