@@ -124,6 +124,20 @@ int lf_get_cut(const lf_handle *h, int32_t *out, int which);
 int lf_get_flow(const lf_handle *h, int64_t *flow);
 /* Device pointer variant of getCut for device-resident consumers. */
 int lf_get_cut_device(const lf_handle *h, int32_t *d_out, int which, void *stream);
+/* LatticeLevelReductions: the final value r of every node (raw units) after the
+ * divide and conquer, i.e. reduction + offset with all offsets reset to 0
+ * (ParametricFlowSolver::run, pf_solver.hpp:33-208; PFFlowNode::r,
+ * pf_flow_node.hpp:233-235).  Nodes of one level set share r up to +-1. */
+int lf_get_levels(const lf_handle *h, int64_t *r_out);
+
+/* ---- total variation: calculate2dTV (solver/tv/tv_solver.hpp:648-681) -- */
+/* Single lambda: calculate2dTV<Kernel, int64>(nx, ny, f, lambda)
+ * (tv_solver.hpp:650-655 -> TVSolver::runSingleLambda :151-176).  f and R are
+ * nx*ny doubles in C order; R is the reference's output (the Python layer of
+ * the reference multiplies it by lambda, pylatticeflow.pyx:437-454).  The
+ * kernel must be 2-D and geocut-applicable (tv_solver.hpp:51-58). */
+int lf_tv_single(const char *kernel, int64_t nx, int64_t ny, const double *f, double lambda,
+                 double *R, lf_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,8 @@ _SIG = {
     "lf_get_cut": (_I, [_P, _P, _I]),
     "lf_get_flow": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "lf_get_cut_device": (_I, [_P, _P, _I, _P]),
+    "lf_get_levels": (_I, [_P, _P]),
+    "lf_tv_single": (_I, [_CP, _I64, _I64, _P, ctypes.c_double, _P, ctypes.POINTER(lf_stats)]),
 }
 
 _lib = None

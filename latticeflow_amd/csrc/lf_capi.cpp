@@ -18,7 +18,7 @@
 #include "../../include/latticeflow_amd.h"
 #include "lf_engine.hpp"
 #include "lf_kernels.hpp"
-#include "lf_levels.hpp"
+#include "lf_tv.hpp"
 
 using namespace lfamd;
 
@@ -79,7 +79,7 @@ struct lf_handle {
   std::vector<double> geocut; // k
   LatticeGeom geom;
   std::unique_ptr<MinCutEngine> eng;
-  std::unique_ptr<LevelSolver> levels;  // LatticeLevelReductions
+  bool levels = false;  // LatticeLevelReductions (parametric divide and conquer)
   lf_tuning tuning{};
   double run_time = 0;
 };
@@ -176,7 +176,7 @@ int lf_create(lf_handle** out, const char* solver, const char* kernel, int ndim,
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
       fail(LF_ERR_DEVICE, "no HIP device available");
     h->eng = make_mincut_engine(g, cap_bytes);
-    if (h->solver == "LatticeLevelReductions") h->levels = make_level_solver(g, h->eng.get());
+    h->levels = h->solver == "LatticeLevelReductions";
     *out = h.release();
   });
 }
@@ -207,7 +207,6 @@ int lf_add_energy_potentials_device(lf_handle* h, const int64_t* du, const int64
   return guarded([&] {
     if (!h || !du || !db) fail(LF_ERR_BAD_ARG, "null argument");
     h->eng->add_energies_device(du, db, reinterpret_cast<hipStream_t>(stream));
-    if (h->levels) h->levels->invalidate();
   });
 }
 
@@ -231,7 +230,6 @@ int lf_add_energy_potentials(lf_handle* h, const int64_t* unary, const int64_t* 
     LF_HIP(hipMemcpyAsync(du, unary, nu * sizeof(int64_t), hipMemcpyHostToDevice, s));
     LF_HIP(hipMemcpyAsync(db, binary, nb * sizeof(int64_t), hipMemcpyHostToDevice, s));
     h->eng->add_energies_device(du, db, s);
-    if (h->levels) h->levels->invalidate();
   });
 }
 
@@ -239,7 +237,6 @@ int lf_reset_energies(lf_handle* h) {
   return guarded([&] {
     if (!h) fail(LF_ERR_BAD_ARG, "null argument");
     h->eng->reset_energies();
-    if (h->levels) h->levels->invalidate();
   });
 }
 
@@ -254,7 +251,7 @@ int lf_run(lf_handle* h) {
   return guarded([&] {
     if (!h) fail(LF_ERR_BAD_ARG, "null argument");
     const auto t0 = std::chrono::steady_clock::now();
-    if (h->levels) h->levels->run(h->tuning);
+    if (h->levels) h->eng->pf_run(h->tuning);
     else h->eng->run(h->tuning);
     h->run_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   });
@@ -270,7 +267,7 @@ int lf_run_time(const lf_handle* h, double* seconds) {
 int lf_get_stats(const lf_handle* h, lf_stats* st) {
   return guarded([&] {
     if (!h || !st) fail(LF_ERR_BAD_ARG, "null argument");
-    *st = h->levels ? h->levels->stats : h->eng->stats;
+    *st = h->eng->stats;
   });
 }
 
@@ -278,8 +275,8 @@ int lf_get_cut_device(const lf_handle* h, int32_t* d_out, int which, void* strea
   return guarded([&] {
     if (!h || !d_out) fail(LF_ERR_BAD_ARG, "null argument");
     if (h->levels) {
-      if (!h->levels->solved()) fail(LF_ERR_STATE, "run() has not been called");
-      h->levels->cut_device(d_out, reinterpret_cast<hipStream_t>(stream));
+      if (!h->eng->pf_solved) fail(LF_ERR_STATE, "run() has not been called");
+      h->eng->pf_cut_device(d_out, reinterpret_cast<hipStream_t>(stream));
       return;
     }
     if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
@@ -303,8 +300,8 @@ int lf_get_cut(const lf_handle* h, int32_t* out, int which) {
       }
     } fr{d, s};
     if (h->levels) {
-      if (!h->levels->solved()) fail(LF_ERR_STATE, "run() has not been called");
-      h->levels->cut_device(d, s);
+      if (!h->eng->pf_solved) fail(LF_ERR_STATE, "run() has not been called");
+      h->eng->pf_cut_device(d, s);
     } else {
       if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
       if (which != LF_CUT_SMIN && which != LF_CUT_SMAX) fail(LF_ERR_BAD_ARG, "bad cut selector");
@@ -315,11 +312,67 @@ int lf_get_cut(const lf_handle* h, int32_t* out, int which) {
   });
 }
 
+int lf_get_levels(const lf_handle* h, int64_t* r_out) {
+  return guarded([&] {
+    if (!h || !r_out) fail(LF_ERR_BAD_ARG, "null argument");
+    if (!h->levels) fail(LF_ERR_STATE, "level values exist only for LatticeLevelReductions");
+    if (!h->eng->pf_solved) fail(LF_ERR_STATE, "run() has not been called");
+    hipStream_t s = h->eng->stream();
+    int64_t* d = nullptr;
+    LF_HIP(hipMallocAsync((void**)&d, sizeof(int64_t) * h->n, s));
+    struct Free {
+      int64_t* d;
+      hipStream_t s;
+      ~Free() {
+        hipFreeAsync(d, s);
+        hipStreamSynchronize(s);
+      }
+    } fr{d, s};
+    h->eng->pf_values_device(d, s);
+    LF_HIP(hipMemcpyAsync(r_out, d, sizeof(int64_t) * h->n, hipMemcpyDeviceToHost, s));
+    LF_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int lf_get_flow(const lf_handle* h, int64_t* flow) {
   return guarded([&] {
     if (!h || !flow) fail(LF_ERR_BAD_ARG, "null argument");
     if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
     *flow = h->eng->flow;
+  });
+}
+
+int lf_tv_single(const char* kernel, int64_t nx, int64_t ny, const double* f, double lambda, double* R,
+                 lf_stats* stats) {
+  return guarded([&] {
+    if (!kernel || !f || !R) fail(LF_ERR_BAD_ARG, "null argument");
+    const KernelDef* kd = find_kernel(kernel);
+    if (!kd) fail(LF_ERR_BAD_NAME, std::string("Bad Kernel or Solver Name: ") + kernel);
+    if (kd->ndim != 2 || !kd->geocut)
+      fail(LF_ERR_BAD_ARG, "Kernel is not valid for GeoCuts or TV Minimization (2-D geocut kernels only).");
+    if (nx <= 0 || ny <= 0) fail(LF_ERR_BAD_ARG, "empty image");
+    const int64_t shape[2] = {nx, ny};
+    lf_handle* h = nullptr;
+    int rc = lf_create(&h, "LatticeLevelReductions", kernel, 2, shape, 8);
+    if (rc) fail(rc, g_last_error);
+    std::unique_ptr<lf_handle> hold(h);
+    TvSetup t = tv_setup(nx, ny, f, lambda, h->offs, h->geocut, h->k);
+    h->eng->set_base_network(t.red.data(), t.alpha.data());
+    const auto t0 = std::chrono::steady_clock::now();
+    h->eng->pf_run(h->tuning);
+    std::vector<int64_t> r((size_t)(nx * ny));
+    hipStream_t s = h->eng->stream();
+    int64_t* d = nullptr;
+    LF_HIP(hipMallocAsync((void**)&d, sizeof(int64_t) * r.size(), s));
+    h->eng->pf_values_device(d, s);
+    LF_HIP(hipMemcpyAsync(r.data(), d, sizeof(int64_t) * r.size(), hipMemcpyDeviceToHost, s));
+    LF_HIP(hipFreeAsync(d, s));
+    LF_HIP(hipStreamSynchronize(s));
+    tv_output(t, r.data(), nx * ny, R);
+    if (stats) {
+      *stats = h->eng->stats;
+      stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
   });
 }
 

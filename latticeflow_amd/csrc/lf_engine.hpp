@@ -49,8 +49,17 @@ class MinCutEngine {
   // Cut of the last solve (LF_CUT_SMIN / LF_CUT_SMAX), as int32 0/1 into device memory.
   virtual void cut_device(int32_t* d_out, int which, hipStream_t s) = 0;
   virtual hipStream_t stream() const = 0;
+  // Parametric divide and conquer over the accumulated network
+  // (ParametricFlowSolver::run, pf_solver.hpp:33-208): final r per node.
+  virtual void pf_run(const lf_tuning& tuning) = 0;
+  virtual void pf_values_device(int64_t* d_out, hipStream_t s) = 0;
+  virtual void pf_cut_device(int32_t* d_out, hipStream_t s) = 0;  // r < 0
+  // Replace the base network (host arrays: red[N], alpha[N*k] node-major, lattice order).
+  virtual void set_base_network(const int64_t* red, const int64_t* alpha) = 0;
   int64_t flow = 0;
   bool solved = false;
+  bool pf_solved = false;
+  int64_t pf_levels = 0, pf_sections = 0;
   lf_stats stats{};
 };
 

@@ -853,6 +853,255 @@ __global__ void k_fill(V* a, V val, long long n) {
   if (v < n) a[v] = val;
 }
 
+// ---- parametric divide and conquer (ParametricFlowSolver, pf_solver.hpp:33-208) ----
+//
+// All regions of one depth of the divide-and-conquer tree are cut in one
+// lattice min-cut: each node carries a region id, arcs between regions are
+// zeroed when a region splits (the reference gives the two parts different
+// keys, push_relabel.hpp:285-303, so those arcs are never used again), and the
+// excess of a node is t(region) - r with t = ceil(mean r) (minus one when it
+// equals max r) (setToMeanReductionPoint, pf_flow_node.hpp:331-396).  A region
+// is final when max r - min r <= 1.  Region sums are exact (128-bit).
+
+struct RegionStat {
+  unsigned long long sum_lo, sum_hi;  // 128-bit two's complement sum of r
+  unsigned long long mn, mx;          // order-preserving encodings of min / max r
+  unsigned long long cnt, cnt1;       // nodes; nodes on the source side after the cut
+  long long thr;                      // threshold of this level
+  int state;                          // 1 active, 2 final
+  int newid;                          // id of the split-off source-side part, -1 when none
+};
+
+struct PFP {
+  long long* r;
+  int* reg;
+  RegionStat* tab;
+  int* nreg;
+  long long N;
+  int D, H, W;
+};
+
+__device__ inline unsigned long long ord64(long long x) { return (unsigned long long)x ^ 0x8000000000000000ull; }
+__device__ inline long long unord64(unsigned long long u) { return (long long)(u ^ 0x8000000000000000ull); }
+
+__device__ inline void add128(RegionStat* t, long long x) {
+  const unsigned long long ux = (unsigned long long)x;
+  const unsigned long long old = atomicAdd(&t->sum_lo, ux);
+  const unsigned long long carry = (old + ux) < old ? 1ull : 0ull;
+  const unsigned long long hi = (x < 0 ? ~0ull : 0ull) + carry;
+  if (hi) atomicAdd(&t->sum_hi, hi);
+}
+
+// correctly rounded conversion of a 128-bit two's complement integer
+__device__ inline double i128_to_double(unsigned long long hi, unsigned long long lo) {
+  const bool neg = (long long)hi < 0;
+  if (neg) {
+    lo = ~lo + 1ull;
+    hi = ~hi + (lo == 0 ? 1ull : 0ull);
+  }
+  double d;
+  if (hi == 0) {
+    d = (double)lo;
+  } else {
+    // keep the top 63 significant bits, fold the rest into a sticky bit, so the
+    // single rounding of the 63-bit conversion is the correct one
+    const int bl = 128 - __clzll((long long)hi);  // 65..128
+    const int sh = bl - 63;                        // 2..65
+    unsigned long long top, sticky;
+    if (sh < 64) {
+      top = (lo >> sh) | (hi << (64 - sh));
+      sticky = (lo & ((1ull << sh) - 1ull)) != 0 ? 1ull : 0ull;
+    } else {
+      top = hi >> (sh - 64);
+      sticky = (lo != 0 || (hi & ((1ull << (sh - 64)) - 1ull)) != 0) ? 1ull : 0ull;
+    }
+    d = ldexp((double)(top | sticky), sh);
+  }
+  return neg ? -d : d;
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_pf_init(PFP f, const long long* __restrict__ red0) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  f.r[v] = red0[v];
+  f.reg[v] = 0;
+  if (v == 0) f.tab[0].state = 1;
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_pf_clear(PFP f, int nreg) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nreg) return;
+  RegionStat& t = f.tab[k];
+  if (t.state != 1) return;
+  t.sum_lo = 0;
+  t.sum_hi = 0;
+  t.mn = ~0ull;
+  t.mx = 0;
+  t.cnt = 0;
+  t.cnt1 = 0;
+  t.newid = -1;
+}
+
+// Per-region aggregation of the active nodes.  MODE 0: sum / min / max / count
+// of r.  MODE 1: count of the nodes with side[v] != 0.  Each block folds its
+// nodes into an LDS hash table keyed by region (spatially coherent, so a block
+// sees few regions) and flushes it with one set of atomics per region.
+template <int MODE>
+__global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __restrict__ side) {
+  constexpr int HSZ = 256;
+  __shared__ int hk[HSZ];
+  __shared__ long long hs[HSZ];
+  __shared__ unsigned long long hmn[HSZ], hmx[HSZ];
+  __shared__ unsigned int hc[HSZ];
+  for (int i = threadIdx.x; i < HSZ; i += NTHREADS) {
+    hk[i] = -1;
+    hs[i] = 0;
+    hmn[i] = ~0ull;
+    hmx[i] = 0;
+    hc[i] = 0;
+  }
+  __syncthreads();
+  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < f.N; v += (long long)gridDim.x * NTHREADS) {
+    const int k = f.reg[v];
+    RegionStat* t = &f.tab[k];
+    if (t->state != 1) continue;
+    const long long x = MODE == 0 ? f.r[v] : (long long)(side[v] != 0);
+    if (MODE == 1 && x == 0) continue;
+    int h = (int)(((unsigned)k * 2654435761u) >> 24) & (HSZ - 1);
+    int slot = -1;
+    for (int probe = 0; probe < 16; ++probe) {
+      const int cur = hk[h];
+      if (cur == k) { slot = h; break; }
+      if (cur == -1) {
+        const int old = atomicCAS(&hk[h], -1, k);
+        if (old == -1 || old == k) { slot = h; break; }
+      }
+      h = (h + 1) & (HSZ - 1);
+    }
+    if (slot >= 0) {
+      if (MODE == 0) {
+        atomicAdd((unsigned long long*)&hs[slot], (unsigned long long)x);
+        atomicMin(&hmn[slot], ord64(x));
+        atomicMax(&hmx[slot], ord64(x));
+      }
+      atomicAdd(&hc[slot], 1u);
+    } else {
+      if (MODE == 0) {
+        add128(t, x);
+        atomicMin(&t->mn, ord64(x));
+        atomicMax(&t->mx, ord64(x));
+        atomicAdd(&t->cnt, 1ull);
+      } else {
+        atomicAdd(&t->cnt1, 1ull);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < HSZ; i += NTHREADS) {
+    const int k = hk[i];
+    if (k < 0 || hc[i] == 0) continue;
+    RegionStat* t = &f.tab[k];
+    if (MODE == 0) {
+      add128(t, hs[i]);
+      atomicMin(&t->mn, hmn[i]);
+      atomicMax(&t->mx, hmx[i]);
+      atomicAdd(&t->cnt, (unsigned long long)hc[i]);
+    } else {
+      atomicAdd(&t->cnt1, (unsigned long long)hc[i]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_pf_thresh(PFP f, int nreg, unsigned long long* n_active) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nreg) return;
+  RegionStat& t = f.tab[k];
+  if (t.state != 1) return;
+  const long long mn = unord64(t.mn), mx = unord64(t.mx);
+  if (mx - mn <= 1) {
+    t.state = 2;
+    return;
+  }
+  const double sum = i128_to_double(t.sum_hi, t.sum_lo);
+  long long thr = (long long)ceil(sum / (double)t.cnt);
+  if (thr == mx) --thr;
+  t.thr = thr;
+  atomicAdd(n_active, 1ull);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_pf_setup(PFP f, KP<T> p, int* err) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const RegionStat& t = f.tab[f.reg[v]];
+  long long e = 0;
+  if (t.state == 1) e = t.thr - f.r[v];
+  if (sizeof(T) == 4 && (e > 0x3fffffffLL || e < -0x3fffffffLL)) atomicOr(err, 1);
+  p.ex[v] = (T)e;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_pf_after(PFP f, KP<T> p, const int* __restrict__ reach) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const RegionStat& t = f.tab[f.reg[v]];
+  if (t.state == 1) f.r[v] = t.thr - (long long)p.ex[v];
+  (void)reach;
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_pf_split(PFP f, int nreg) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nreg) return;
+  RegionStat& t = f.tab[k];
+  if (t.state != 1) return;
+  if (t.cnt1 > 0 && t.cnt1 < t.cnt) {
+    // every region is nonempty, so ids stay below N; the test only guards memory
+    const int nid = atomicAdd(f.nreg, 1);
+    if (nid >= f.N) {
+      t.newid = -1;
+      return;
+    }
+    t.newid = nid;
+    f.tab[nid].state = 1;
+  } else {
+    t.newid = -1;
+  }
+}
+
+// source-side nodes of a split region move to the new region
+__global__ __launch_bounds__(NTHREADS) void k_pf_move(PFP f, const int* __restrict__ reach) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const int k = f.reg[v];
+  const RegionStat& t = f.tab[k];
+  if (t.state == 1 && t.newid >= 0 && reach[v]) f.reg[v] = t.newid;
+}
+
+// zero every arc whose ends lie in different regions
+template <int K, typename T>
+__global__ __launch_bounds__(NTHREADS) void k_pf_cutarcs(PFP f, KP<T> p) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const int x = (int)(v % p.W), y = (int)((v / p.W) % p.H), z = (int)(v / ((long long)p.W * p.H));
+  const int k = f.reg[v];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int nz = z + p.dirs.dz[i], ny = y + p.dirs.dy[i], nx = x + p.dirs.dx[i];
+    if (nz < 0 || nz >= p.D || ny < 0 || ny >= p.H || nx < 0 || nx >= p.W) continue;
+    const long long w = ((long long)nz * p.H + ny) * p.W + nx;
+    if (f.reg[w] != k) {
+      T* a = &p.res[(long long)i * p.N + v];
+      if (*a != 0) *a = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_pf_cut(PFP f, int* __restrict__ out) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  out[v] = f.r[v] < 0 ? 1 : 0;
+}
+
 inline unsigned blocks_for(long long n) { return (unsigned)((n + NTHREADS - 1) / NTHREADS); }
 
 // ---- engine -----------------------------------------------------------------------
@@ -908,6 +1157,7 @@ class Engine final : public MinCutEngine {
     hipFree(red0_); hipFree(alpha0_); hipFree(res_); hipFree(mbox_); hipFree(ex_); hipFree(h_);
     hipFree(mask_); hipFree(reach_); hipFree(flags_[0]); hipFree(flags_[1]); hipFree(bflags_[0]); hipFree(bflags_[1]); hipFree(dev_small_); hipFree(part_);
     hipHostFree(host_small_);
+    hipFree(pf_r_); hipFree(pf_reg_); hipFree(pf_tab_); hipFree(pf_nreg_);
     hipEventDestroy(ev_[0]);
     hipEventDestroy(ev_[1]);
     hipStreamDestroy(s_);
@@ -942,13 +1192,9 @@ class Engine final : public MinCutEngine {
     solved = false;
   }
 
-  void run(const lf_tuning& tun) override {
-    using clk = std::chrono::steady_clock;
-    const auto t0 = clk::now();
-    stats = lf_stats{};
+  // working state (res, ex, zero mailbox) from the base network, with range checks
+  void load_working() {
     KP<T> p = params();
-    p.iters = tun.local_iters > 0 ? tun.local_iters : 4;
-    const int R = tun.rounds_per_relabel > 0 ? tun.rounds_per_relabel : 4;
     int* err = reinterpret_cast<int*>(dev_small_);
     unsigned long long* mx = reinterpret_cast<unsigned long long*>(dev_small_ + 32);
     LF_HIP(hipMemsetAsync(dev_small_, 0, 64, s_));
@@ -959,26 +1205,35 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipGetLastError());
     LF_HIP(hipMemcpyAsync(host_small_, dev_small_, 64, hipMemcpyDeviceToHost, s_));
     LF_HIP(hipStreamSynchronize(s_));
-    {
-      const int e = reinterpret_cast<int*>(host_small_)[0];
-      const unsigned long long* m = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(host_small_) + 32);
-      if (e & 2) fail(LF_ERR_NONSUBMODULAR, "negative pairwise capacity");
-      const double bound = (double)m[0] + (double)K * (double)m[1] + 2.0 * (double)m[1];
-      const double lim = sizeof(T) == 4 ? 2147483647.0 : 9.2e18;
-      if (bound >= lim)
-        fail(LF_ERR_OVERFLOW, sizeof(T) == 4 ? "capacities exceed the int32 range; use cap_bytes=8"
-                                             : "capacities exceed the int64 range");
-    }
-    double t_push = 0, t_bfs = 0;
-    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, tiles_ptr(), 0ull, 1LL);
+    const int e = reinterpret_cast<int*>(host_small_)[0];
+    const unsigned long long* m = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(host_small_) + 32);
+    if (e & 2) fail(LF_ERR_NONSUBMODULAR, "negative pairwise capacity");
+    max_red_ = m[0];
+    max_alpha_ = m[1];
+    const double bound = (double)m[0] + (double)K * (double)m[1] + 2.0 * (double)m[1];
+    const double lim = sizeof(T) == 4 ? 2147483647.0 : 9.2e18;
+    if (bound >= lim)
+      fail(LF_ERR_OVERFLOW, sizeof(T) == 4 ? "capacities exceed the int32 range; use cap_bytes=8"
+                                           : "capacities exceed the int64 range");
+  }
+
+  // Maximum preflow of the current working state (res, ex), then S_min.
+  // Accumulates into stats.
+  void solve_state(const lf_tuning& tun) {
+    using clk = std::chrono::steady_clock;
+    KP<T> p = params();
+    p.iters = tun.local_iters > 0 ? tun.local_iters : 4;
+    const int R = tun.rounds_per_relabel > 0 ? tun.rounds_per_relabel : 4;
     bool pushed = false;
+    long long cycles = 0;
     for (;;) {
       const auto tb = clk::now();
       const long long active = global_relabel(p, pushed);
-      t_bfs += std::chrono::duration<double>(clk::now() - tb).count();
+      stats.bfs_seconds += std::chrono::duration<double>(clk::now() - tb).count();
       ++stats.cycles;
+      ++cycles;
       if (active == 0) break;
-      if (tun.max_cycles > 0 && stats.cycles > tun.max_cycles)
+      if (tun.max_cycles > 0 && cycles > tun.max_cycles)
         fail(LF_ERR_STATE, "max_cycles reached before the flow became maximum");
       LF_HIP(hipEventRecord(ev_[0], s_));
       for (int r = 0; r < R; ++r) {
@@ -995,29 +1250,118 @@ class Engine final : public MinCutEngine {
       LF_HIP(hipEventSynchronize(ev_[1]));
       float ms = 0;
       LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-      t_push += ms * 1e-3;
+      stats.push_seconds += ms * 1e-3;
     }
     const auto tf = clk::now();
-    // S_min flood
     hipLaunchKernelGGL((k_flood_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, reach_);
     run_tiled([&](const BP& q) {
       hipLaunchKernelGGL((k_flood_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, reach_);
       ++stats.flood_launches;
     });
-    stats.flood_seconds = std::chrono::duration<double>(clk::now() - tf).count();
-    // flow value
-    hipLaunchKernelGGL((k_flow<T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, p, red0_, part_);
-    hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
-    flow = (long long)read_counter();
-    LF_HIP(hipStreamSynchronize(s_));
-    stats.push_seconds = t_push;
-    stats.bfs_seconds = t_bfs;
+    stats.flood_seconds += std::chrono::duration<double>(clk::now() - tf).count();
+  }
+
+  void begin_stats() {
+    stats = lf_stats{};
+    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, tiles_ptr(), 0ull, 1LL);
+  }
+  void end_stats() {
     stats.tile_nodes = G::NT;
     stats.bytes_per_node = 2 * ((long long)K * sizeof(T) + sizeof(T) + 4);
     LF_HIP(hipMemcpy(host_small_, tiles_ptr(), sizeof(unsigned long long), hipMemcpyDeviceToHost));
     stats.tiles_pushed = (long long)*reinterpret_cast<unsigned long long*>(host_small_);
+  }
+
+  void run(const lf_tuning& tun) override {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    begin_stats();
+    load_working();
+    solve_state(tun);
+    KP<T> p = params();
+    hipLaunchKernelGGL((k_flow<T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, p, red0_, part_);
+    hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
+    flow = (long long)read_counter();
+    LF_HIP(hipStreamSynchronize(s_));
+    end_stats();
     stats.seconds = std::chrono::duration<double>(clk::now() - t0).count();
     solved = true;
+    pf_solved = false;
+  }
+
+  // ---- parametric divide and conquer (LatticeLevelReductions / TV) ----------
+  void pf_run(const lf_tuning& tun) override {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    begin_stats();
+    load_working();
+    ensure_pf_buffers();
+    KP<T> p = params();
+    PFP pf = pf_params();
+    // one region holding every node
+    hipLaunchKernelGGL((k_fill<char>), dim3(blocks_for((long long)sizeof(RegionStat) * N_)), dim3(NTHREADS), 0, s_,
+                       reinterpret_cast<char*>(pf.tab), (char)0, (long long)sizeof(RegionStat) * N_);
+    hipLaunchKernelGGL((k_pf_init), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, red0_);
+    hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(NTHREADS), 0, s_, pf.nreg, 1, 1LL);
+    long long levels = 0, sections = 0;
+    for (;;) {
+      // region statistics and thresholds (setToMeanReductionPoint)
+      const int nreg = (int)std::min<long long>(read_int(pf.nreg), N_);
+      hipLaunchKernelGGL((k_pf_clear), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg);
+      hipLaunchKernelGGL((k_pf_stats_impl<0>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)nullptr);
+      zero_counter();
+      hipLaunchKernelGGL((k_pf_thresh), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, counter_ptr());
+      LF_HIP(hipGetLastError());
+      const long long active = (long long)read_counter();
+      if (active == 0) break;
+      ++levels;
+      sections += active;
+      // section min-cut of every active region at once
+      int* err = reinterpret_cast<int*>(dev_small_);
+      LF_HIP(hipMemsetAsync(err, 0, sizeof(int), s_));
+      hipLaunchKernelGGL((k_pf_setup<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, err);
+      LF_HIP(hipMemcpyAsync(host_small_, err, sizeof(int), hipMemcpyDeviceToHost, s_));
+      LF_HIP(hipStreamSynchronize(s_));
+      if (*reinterpret_cast<int*>(host_small_))
+        fail(LF_ERR_OVERFLOW, "parametric offsets exceed the capacity width; use cap_bytes=8");
+      solve_state(tun);
+      // r from the flow, split by S_min, cut the arcs between the new regions
+      hipLaunchKernelGGL((k_pf_after<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, reach_);
+      hipLaunchKernelGGL((k_pf_stats_impl<1>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)reach_);
+      hipLaunchKernelGGL((k_pf_split), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg);
+      hipLaunchKernelGGL((k_pf_move), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, (const int*)reach_);
+      hipLaunchKernelGGL((k_pf_cutarcs<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p);
+      LF_HIP(hipGetLastError());
+    }
+    LF_HIP(hipStreamSynchronize(s_));
+    end_stats();
+    stats.reserved = levels;
+    pf_levels = levels;
+    pf_sections = sections;
+    stats.seconds = std::chrono::duration<double>(clk::now() - t0).count();
+    pf_solved = true;
+    solved = false;
+  }
+
+  void pf_values_device(int64_t* d_out, hipStream_t s) override {
+    LF_HIP(hipMemcpyAsync(d_out, pf_r_, sizeof(long long) * N_, hipMemcpyDeviceToDevice, s_));
+    sync_to(s);
+  }
+  void pf_cut_device(int32_t* d_out, hipStream_t s) override {
+    hipLaunchKernelGGL((k_pf_cut), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf_params(), d_out);
+    LF_HIP(hipGetLastError());
+    sync_to(s);
+  }
+  void set_base_network(const int64_t* red, const int64_t* alpha_lattice_major) override {
+    // host arrays: red[N], alpha[N*K] (node-major, lattice direction order)
+    std::vector<long long> planes((size_t)N_ * K);
+    for (long long v = 0; v < N_; ++v)
+      for (int i = 0; i < K; ++i) planes[(size_t)i * N_ + v] = alpha_lattice_major[(size_t)v * K + i];
+    LF_HIP(hipMemcpyAsync(red0_, red, sizeof(long long) * N_, hipMemcpyHostToDevice, s_));
+    LF_HIP(hipMemcpyAsync(alpha0_, planes.data(), sizeof(long long) * N_ * K, hipMemcpyHostToDevice, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    solved = false;
+    pf_solved = false;
   }
 
   void cut_device(int32_t* d_out, int which, hipStream_t s) override {
@@ -1048,6 +1392,35 @@ class Engine final : public MinCutEngine {
   unsigned long long* tiles_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 24); }
   void zero_counter() {
     hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, counter_ptr(), 0ull, 1LL);
+  }
+  int read_int(const int* d) {
+    LF_HIP(hipMemcpyAsync(host_small_, d, sizeof(int), hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    return *reinterpret_cast<int*>(host_small_);
+  }
+  void sync_to(hipStream_t s) {
+    if (s && s != s_) {
+      hipEvent_t ev;
+      LF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      LF_HIP(hipEventRecord(ev, s_));
+      LF_HIP(hipStreamWaitEvent(s, ev, 0));
+      LF_HIP(hipEventDestroy(ev));
+    } else {
+      LF_HIP(hipStreamSynchronize(s_));
+    }
+  }
+  void ensure_pf_buffers() {
+    if (pf_r_) return;
+    LF_HIP(hipMalloc(&pf_r_, sizeof(long long) * N_));
+    LF_HIP(hipMalloc(&pf_reg_, sizeof(int) * N_));
+    LF_HIP(hipMalloc(&pf_tab_, sizeof(RegionStat) * N_));
+    LF_HIP(hipMalloc(&pf_nreg_, sizeof(int) * 2));
+  }
+  PFP pf_params() const {
+    PFP f;
+    f.r = pf_r_; f.reg = pf_reg_; f.tab = pf_tab_; f.nreg = pf_nreg_;
+    f.N = N_; f.D = p_.D; f.H = p_.H; f.W = p_.W;
+    return f;
   }
   unsigned long long read_counter() {
     LF_HIP(hipMemcpyAsync(host_small_, counter_ptr(), sizeof(unsigned long long), hipMemcpyDeviceToHost, s_));
@@ -1123,6 +1496,11 @@ class Engine final : public MinCutEngine {
   long long* part_ = nullptr;
   void* host_small_ = nullptr;
   hipEvent_t ev_[2] = {nullptr, nullptr};
+  unsigned long long max_red_ = 0, max_alpha_ = 0;
+  long long* pf_r_ = nullptr;
+  int* pf_reg_ = nullptr;
+  RegionStat* pf_tab_ = nullptr;
+  int* pf_nreg_ = nullptr;
 };
 
 template <int K, class G, class B>

@@ -229,3 +229,30 @@ class LatticeFlow(object):
         f = ctypes.c_int64()
         check(lib().lf_get_flow(self._h, ctypes.byref(f)))
         return f.value
+
+    def levels(self):
+        """LatticeLevelReductions: final value r of every node (raw units), array of ``shape``."""
+        out = np.empty(self.n_nodes, dtype=np.int64)
+        check(lib().lf_get_levels(self._h, out.ctypes.data))
+        return out.reshape(*self._dimensions)
+
+
+def _tv_single(X, lam, kernel):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    if X.ndim != 2:
+        raise ValueError("calculate2dTV needs a 2-d array")
+    nx, ny = X.shape
+    R = np.empty((nx, ny), dtype=np.float64)
+    st = _lib.lf_stats()
+    check(lib().lf_tv_single(kernel.encode(), nx, ny, X.ctypes.data, float(lam), R.ctypes.data,
+                             ctypes.byref(st)))
+    return R, st.as_dict()
+
+
+def calculate2dTV(X, lam, kernel="Star2d_16", return_stats=False):
+    """pyx:437-454: exact 2-D TV of X at one lambda, times lambda (as the reference's
+    Python layer returns it).  ``kernel`` (new, keyword) selects the geocut
+    kernel; the reference hard-wires Star2d_16."""
+    R, st = _tv_single(X, lam, kernel)
+    R = R * lam
+    return (R, st) if return_stats else R

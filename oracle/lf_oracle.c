@@ -35,6 +35,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 
 #define ORC_MAXK 128
 #define ORC_MAXD 4
@@ -695,5 +696,227 @@ int orc_dinic(int nd, const int64_t *shape, int k, const int32_t *offs,
   for (int64_t v = 0; v < N; ++v) red_out[v] = snk[v] > 0 ? snk[v] : -src[v];
   free(nbr); free(src); free(srcb); free(snk); free(snkb); free(lev); free(it); free(q);
   free(path); free(parc);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Parametric divide and conquer (ParametricFlowSolver::run,                 */
+/* solver/parametricflow/pf_solver.hpp:33-208) on PFFlowNode                  */
+/* (solver/parametricflow/pf_flow_node.hpp:83-421).                           */
+/*                                                                           */
+/* Per node: r() = reduction + current_fv_offset.  Each range is set to the  */
+/* threshold ceil(mean r) (minus one when it equals max r) by setOffset, cut */
+/* by a keyed PRFlow section, split in place by the state bit, the "on" part  */
+/* flipped back, and both parts queued (FIFO).  A range stops when            */
+/* max r - min r <= 1 (setToMeanReductionPoint, pf_flow_node.hpp:331-396).    */
+
+typedef struct { int64_t start, end; } range_t;
+
+/* pf_flow_node.hpp:331-396; returns 1 when the range is at its mean point */
+static int pf_set_to_mean(pr_t *p, int64_t *off, const int64_t *pts, int64_t s, int64_t e) {
+  __int128 r_sum = 0;
+  double count = 0;
+  int64_t r_min = p->red[pts[s]] + off[pts[s]], r_max = r_min;
+  for (int64_t i = s; i < e; ++i) {
+    int64_t n = pts[i];
+    int64_t r = p->red[n] + off[n];
+    r_sum += r;
+    if (r < r_min) r_min = r;
+    if (r > r_max) r_max = r;
+    count += 1;
+  }
+  if (r_max - r_min <= 1) return 1;
+  int64_t r_new = (int64_t)ceil((double)r_sum / count);
+  if (r_new == r_max) --r_new;
+  for (int64_t i = s; i < e; ++i) {
+    int64_t n = pts[i];
+    /* setOffset (pf_flow_node.hpp:311-319): reduction += old_offset - new_offset */
+    p->red[n] += off[n] - r_new;
+    off[n] = r_new;
+  }
+  return 0;
+}
+
+/* NetworkFlowNode::flipNode<1> (network_flow_node.hpp:313-354) */
+static void pr_flip_back_raw(pr_t *p, int64_t n) {
+  for (int i = 0; i < p->k; ++i) p->alpha[(n + p->jumps[i]) * p->k + p->rev[i]] *= -1;
+  p->key_state[n] &= ~1u;
+}
+
+static int64_t pf_run(pr_t *p, int64_t *off, int64_t *pts, int64_t npts, int64_t *bound_out, int64_t *nbound) {
+  range_t *q = (range_t *)malloc(sizeof(range_t) * (size_t)(2 * npts + 4));
+  int64_t qh = 0, qt = 0, nb = 0;
+  unsigned key = 1;
+  q[qt++] = (range_t){0, npts};
+  bound_out[nb++] = 0;
+  int64_t sections = 0;
+  while (qh < qt) {
+    range_t rg = q[qh++];
+    if (pf_set_to_mean(p, off, pts, rg.start, rg.end)) continue;
+    ++sections;
+    /* prepareSection (push_relabel.hpp:1360-1381): setKey<0> */
+    for (int64_t i = rg.start; i < rg.end; ++i) p->key_state[pts[i]] = (key << 1) | 0u;
+    /* runSection (:1383-1463) */
+    p->key = key << 1;
+    pr_hot_start(p, pts + rg.start, rg.end - rg.start);
+    pr_quick_flow(p);
+    pr_run_loop(p);
+    /* in-place separation by state (pf_solver.hpp:101-152) */
+    int64_t fwd = rg.start, bck = rg.end - 1, mid = 0;
+#define ST(i_) ((int)(p->key_state[pts[(i_)]] & 1u))
+    for (;;) {
+      int done = 0;
+      while (ST(fwd) == 0) {
+        ++fwd;
+        if (bck == fwd) { done = 1; break; }
+      }
+      if (!done) {
+        while (ST(bck) == 1) {
+          --bck;
+          if (bck == fwd) { done = 1; break; }
+        }
+      }
+      if (!done) {
+        int64_t t = pts[fwd]; pts[fwd] = pts[bck]; pts[bck] = t;
+        ++fwd;
+        if (fwd == bck) done = 1;
+      }
+      if (done) {
+        if (bck != rg.end && ST(bck) == 0) ++bck;
+        mid = bck - rg.start;
+        break;
+      }
+    }
+#undef ST
+    int64_t top = rg.end - rg.start;
+    if (mid > 0) q[qt++] = (range_t){rg.start, rg.start + mid};
+    if (mid < top) {
+      q[qt++] = (range_t){rg.start + mid, rg.end};
+      for (int64_t i = rg.start + mid; i < rg.end; ++i) pr_flip_back_raw(p, pts[i]);
+    }
+    if (mid > 0 && mid < top) bound_out[nb++] = mid + rg.start;
+    /* cleanupSection (:1468-1487) */
+    for (int64_t i = rg.start; i < rg.end; ++i) {
+      int64_t n = pts[i];
+      if (p->key_state[n] & 1u) pr_flip_back_raw(p, n);
+      p->key_state[n] &= 1u;
+    }
+    ++key;
+    if (qt + 2 > 2 * npts + 4) { /* cannot happen: every queued range is nonempty */
+      free(q);
+      return -1;
+    }
+  }
+  /* reset every offset to 0 (pf_solver.hpp:181-182) */
+  for (int64_t i = 0; i < npts; ++i) {
+    int64_t n = pts[i];
+    p->red[n] += off[n];
+    off[n] = 0;
+  }
+  *nbound = nb;
+  free(q);
+  return sections;
+}
+
+/* LatticeLevelReductions on the network (red, alpha): final r per node, the
+ * level-set index of each node (sorted boundaries), and the number of
+ * section solves. */
+int orc_pf_solve(int nd, const int64_t *shape, int k, const int32_t *offs,
+                 const int64_t *red_in, const int64_t *alpha_in,
+                 int64_t *r_out, int64_t *level_out, int64_t *stats) {
+  ulat_t L;
+  if (ulat_init(&L, nd, shape, k, offs)) return -1;
+  pr_t p;
+  int64_t pstride[ORC_MAXD] = {0}, eb[ORC_MAXD] = {0};
+  if (pr_build(&p, &L, pstride, eb)) { pr_free(&p); return -3; }
+  int64_t *map = (int64_t *)malloc(sizeof(int64_t) * (L.n + 1));
+  int64_t *pts = (int64_t *)malloc(sizeof(int64_t) * (L.n + 1));
+  int64_t *off = (int64_t *)calloc((size_t)p.np, sizeof(int64_t));
+  int64_t *bnd = (int64_t *)malloc(sizeof(int64_t) * (L.n + 2));
+  for (int64_t v = 0; v < L.n; ++v) {
+    int64_t w = map[v] = padded_index(&L, pstride, p.begin, v);
+    pts[v] = w;
+    p.red[w] = red_in[v];
+    for (int i = 0; i < k; ++i) p.alpha[w * k + i] = alpha_in[v * k + i];
+  }
+  int64_t nb = 0;
+  int64_t sections = L.n > 0 ? pf_run(&p, off, pts, L.n, bnd, &nb) : 0;
+  if (sections < 0) return -4;
+  /* level index per node: sort boundaries, map ranges */
+  for (int64_t a = 1; a < nb; ++a) {
+    int64_t x = bnd[a], b = a - 1;
+    while (b >= 0 && bnd[b] > x) { bnd[b + 1] = bnd[b]; --b; }
+    bnd[b + 1] = x;
+  }
+  bnd[nb] = L.n;
+  int64_t *lev_pad = (int64_t *)calloc((size_t)p.np, sizeof(int64_t));
+  for (int64_t l = 0; l < nb; ++l)
+    for (int64_t i = bnd[l]; i < bnd[l + 1]; ++i) lev_pad[pts[i]] = l;
+  for (int64_t v = 0; v < L.n; ++v) {
+    r_out[v] = p.red[map[v]];
+    if (level_out) level_out[v] = lev_pad[map[v]];
+  }
+  if (stats) { stats[0] = sections; stats[1] = nb; }
+  free(lev_pad); free(map); free(pts); free(off); free(bnd); pr_free(&p);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Single-lambda TV (TVSolver::setup + runSingleLambda,                      */
+/* solver/tv/tv_solver.hpp:71-112,151-176; calculate2dTV :650-655), int64:    */
+/* function precision 40 bits (pf_flow_node.hpp:103-115).                     */
+
+#define TV_FBITS 40
+
+static int64_t tv_to_fv(double v) { return (int64_t)round(v * (double)((int64_t)1 << TV_FBITS)); }
+
+/* shape (nx, ny); offs/w: k lattice-order directions and geocut weights.
+ * Writes the reference's R (before the Python layer's factor lambda). */
+int orc_tv_single(int64_t nx, int64_t ny, int k, const int32_t *offs, const double *w,
+                  const double *f, double lambda, double *R, int64_t *stats) {
+  const int64_t n = nx * ny;
+  if (n <= 0) return -1;
+  double v_min = f[0], v_max = f[0], v_sum = f[0];
+  for (int64_t i = 1; i < n; ++i) {
+    double v = f[i];
+    v_min = v_min < v ? v_min : v;
+    v_max = v_max > v ? v_max : v;
+    v_sum += v;
+  }
+  const double M = v_sum / (double)n;
+  v_min -= M;
+  v_max -= M;
+  const double S = 2 * (v_max > -v_min ? v_max : -v_min);
+  const double max_lambda = lambda > 1e-8 ? lambda : 1e-8;
+  int64_t shape[2] = {nx, ny};
+  int64_t *red = (int64_t *)calloc((size_t)n, sizeof(int64_t));
+  int64_t *alpha = (int64_t *)calloc((size_t)(n * k), sizeof(int64_t));
+  int64_t *r = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t uf = tv_to_fv(0.5 + (f[i] - M) / S);
+    red[i] += 2 * uf; /* addE1(0, uf) */
+  }
+  const double gsv = 0.5 / (S * max_lambda);
+  ulat_t L;
+  if (ulat_init(&L, 2, shape, k, offs)) { free(red); free(alpha); free(r); return -1; }
+  for (int64_t v = 0; v < n; ++v)
+    for (int e = 0; e < k / 2; ++e) {
+      int64_t u = ulat_nbr(&L, v, e);
+      if (u < 0) continue;
+      int64_t pwf = tv_to_fv((gsv * w[e]) / S); /* toUnshiftedFVDType */
+      /* addE2(0, pwf, pwf, 0): q1 = q2 = pwf, q12 = -2 pwf */
+      alpha[v * k + e] += 2 * pwf;
+      alpha[u * k + rev_dir(k, e)] += 2 * pwf;
+    }
+  int rc = orc_pf_solve(2, shape, k, offs, red, alpha, r, NULL, stats);
+  if (rc) { free(red); free(alpha); free(r); return rc; }
+  double sum = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    double v = ((double)(r[i] / 2) / (double)((int64_t)1 << TV_FBITS) - 0.5) * S + M;
+    R[i] = v;
+    sum += v;
+  }
+  for (int64_t i = 0; i < n; ++i) R[i] = (R[i] - sum / (double)n) * S + M;
+  free(red); free(alpha); free(r);
   return 0;
 }

@@ -160,7 +160,12 @@ def lib():
         L.orc_prflow_solve.argtypes = [ci, i64p, ci, i32p, i64p, i64p, i64p, i64p, u8p, i64p]
         L.orc_dinic.argtypes = [ci, i64p, ci, i32p, i64p, i64p, i64p, i64p, i64p]
         L.orc_cuts.argtypes = [ci, i64p, ci, i32p, i64p, i64p, i64p, i64p, u8p, u8p]
-        for f in (L.orc_reparam, L.orc_prflow_solve, L.orc_dinic, L.orc_cuts):
+        L.orc_pf_solve.argtypes = [ci, i64p, ci, i32p, i64p, i64p, i64p, i64p, i64p]
+        f64p = ctypes.POINTER(ctypes.c_double)
+        L.orc_tv_single.argtypes = [ctypes.c_int64, ctypes.c_int64, ci, i32p, f64p, f64p,
+                                    ctypes.c_double, f64p, i64p]
+        for f in (L.orc_reparam, L.orc_prflow_solve, L.orc_dinic, L.orc_cuts, L.orc_pf_solve,
+                  L.orc_tv_single):
             f.restype = ci
         _LIB = L
     return _LIB
@@ -247,6 +252,144 @@ def dinic(shape, offs, red, alpha):
     f2, smin, smax = cuts(shape, offs, red, red_o, res_o)
     assert f2 == int(flow[0])
     return dict(flow=int(flow[0]), smin=smin, smax=smax, red=red_o, res=res_o)
+
+
+def pf_solve(shape, offs, red, alpha):
+    """ParametricFlowSolver restatement. Returns dict(r, level, sections, n_levels)."""
+    shape, offs, k, nd = _lat(shape, offs)
+    n = int(np.prod(shape))
+    red = np.ascontiguousarray(red, np.int64)
+    alpha = np.ascontiguousarray(alpha, np.int64)
+    r = np.zeros(n, np.int64)
+    lev = np.zeros(n, np.int64)
+    st = np.zeros(2, np.int64)
+    rc = lib().orc_pf_solve(nd, _p(shape, ctypes.c_int64), k, _p(offs, ctypes.c_int32),
+                            _p(red, ctypes.c_int64), _p(alpha, ctypes.c_int64),
+                            _p(r, ctypes.c_int64), _p(lev, ctypes.c_int64), _p(st, ctypes.c_int64))
+    assert rc == 0, rc
+    return dict(r=r, level=lev, sections=int(st[0]), n_levels=int(st[1]))
+
+
+def tv_single(f, lam, kernel="Full2d_8"):
+    """TVSolver::runSingleLambda restatement (before the Python layer's factor lambda)."""
+    f = np.ascontiguousarray(f, np.float64)
+    nx, ny = f.shape
+    offs, w = edge_order(kernel, (nx, ny))
+    offs = np.ascontiguousarray(offs, np.int32)
+    w = np.ascontiguousarray(w, np.float64)
+    R = np.zeros(f.size, np.float64)
+    st = np.zeros(2, np.int64)
+    rc = lib().orc_tv_single(nx, ny, int(offs.shape[0]), _p(offs, ctypes.c_int32),
+                             _p(w, ctypes.c_double), _p(f, ctypes.c_double), float(lam),
+                             _p(R, ctypes.c_double), _p(st, ctypes.c_int64))
+    assert rc == 0, rc
+    return R.reshape(nx, ny), dict(sections=int(st[0]), n_levels=int(st[1]))
+
+
+def tv1d_condat(y, lam):
+    """Exact 1-D TV denoising argmin 1/2 sum (x - y)^2 + lam sum |x_{i+1} - x_i|
+    (L. Condat, "A direct algorithm for 1D total variation denoising", 2013)."""
+    y = np.asarray(y, np.float64)
+    N = len(y)
+    x = np.empty(N)
+    if N == 0:
+        return x
+    k = k0 = kmin = kplus = 0
+    vmin = y[0] - lam
+    vmax = y[0] + lam
+    umin = lam
+    umax = -lam
+    while True:
+        while k == N - 1:
+            if umin < 0.0:
+                while True:
+                    x[k0] = vmin
+                    k0 += 1
+                    if k0 > kmin:
+                        break
+                k = kmin = k0
+                vmin = y[k]
+                umin = lam
+                umax = vmin + umin - vmax
+            elif umax > 0.0:
+                while True:
+                    x[k0] = vmax
+                    k0 += 1
+                    if k0 > kplus:
+                        break
+                k = kplus = k0
+                vmax = y[k]
+                umax = -lam
+                umin = vmax + umax - vmin
+            else:
+                vmin += umin / (k - k0 + 1)
+                while True:
+                    x[k0] = vmin
+                    k0 += 1
+                    if k0 > k:
+                        return x
+        umin += y[k + 1] - vmin
+        umax += y[k + 1] - vmax
+        if umin < -lam:
+            while True:
+                x[k0] = vmin
+                k0 += 1
+                if k0 > kmin:
+                    break
+            k = kmin = kplus = k0
+            vmin = y[k]
+            vmax = y[k] + 2 * lam
+            umin = lam
+            umax = -lam
+        elif umax > lam:
+            while True:
+                x[k0] = vmax
+                k0 += 1
+                if k0 > kplus:
+                    break
+            k = kmin = kplus = k0
+            vmax = y[k]
+            vmin = y[k] - 2 * lam
+            umin = lam
+            umax = -lam
+        else:
+            k += 1
+            if umin >= lam:
+                vmin += (umin - lam) / (k - k0 + 1)
+                umin = lam
+                kmin = k
+            if umax <= -lam:
+                vmax += (umax + lam) / (k - k0 + 1)
+                umax = -lam
+                kplus = k
+
+
+def rof_dual(y, shape, offs, c):
+    """argmin_x 1/2 |x - y|^2 + sum_e c_e |x_i - x_j| over the undirected lattice
+    edges (first k/2 directions), via its box-constrained dual solved with
+    L-BFGS-B: x = y - D^T u, |u_e| <= c_e.  For small lattices."""
+    from scipy.optimize import minimize
+    y = np.asarray(y, np.float64).reshape(-1)
+    pairs = _pairs(shape, np.asarray(offs))
+    rows_i, rows_j, cs = [], [], []
+    for e, v, w in pairs:
+        rows_i.append(v); rows_j.append(w); cs.append(np.full(len(v), c[e] if np.ndim(c) else c))
+    I = np.concatenate(rows_i); J = np.concatenate(rows_j); C = np.concatenate(cs)
+    m = len(I)
+
+    def xof(u):
+        x = y.copy()
+        np.subtract.at(x, I, u)
+        np.add.at(x, J, u)
+        return x
+
+    def fg(u):
+        x = xof(u)
+        g = -(x[I] - x[J])
+        return 0.5 * float(x @ x), g
+    res = minimize(fg, np.zeros(m), jac=True, method="L-BFGS-B", bounds=list(zip(-C, C)),
+                   options=dict(ftol=1e-16, gtol=1e-13, maxiter=200000, maxcor=50))
+    return xof(res.x)
 
 
 # ----------------------------------------------------------------------------
