@@ -138,6 +138,14 @@ int lf_get_levels(const lf_handle *h, int64_t *r_out);
  * kernel must be 2-D and geocut-applicable (tv_solver.hpp:51-58). */
 int lf_tv_single(const char *kernel, int64_t nx, int64_t ny, const double *f, double lambda,
                  double *R, lf_stats *stats);
+/* Regularization path at the given lambdas: calculate2dTV<Kernel, int64>(nx, ny,
+ * f, vector<double> lambda) (tv_solver.hpp:659-681 -> TVSolver::run +
+ * getRegularizationPath :181-244).  R is n_lambda * nx * ny doubles, slice l
+ * for lambdas[l] (the reference's Python layer divides every slice by the last
+ * lambda of the list, pylatticeflow.pyx:457-484).  All lambdas are solved
+ * together on the device as layers of one stacked lattice. */
+int lf_tv_path(const char *kernel, int64_t nx, int64_t ny, const double *f, int64_t n_lambda,
+               const double *lambdas, double *R, lf_stats *stats);
 
 #ifdef __cplusplus
 }

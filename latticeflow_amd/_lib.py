@@ -86,6 +86,7 @@ _SIG = {
     "lf_get_cut_device": (_I, [_P, _P, _I, _P]),
     "lf_get_levels": (_I, [_P, _P]),
     "lf_tv_single": (_I, [_CP, _I64, _I64, _P, ctypes.c_double, _P, ctypes.POINTER(lf_stats)]),
+    "lf_tv_path": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.POINTER(lf_stats)]),
 }
 
 _lib = None

@@ -342,37 +342,118 @@ int lf_get_flow(const lf_handle* h, int64_t* flow) {
   });
 }
 
+// Exact TV of one image at several lambdas: the lambdas are the layers of one
+// stacked lattice (no arcs between layers), solved by one device divide and
+// conquer.  r_out[l*n + i] receives the final raw r of pixel i at lambda l.
+static void tv_solve_layers(const KernelDef& kd, int64_t nx, int64_t ny, const TvBase& b,
+                            const std::vector<double>& lams, const lf_tuning& tun, std::vector<int64_t>& r_out,
+                            lf_stats* stats) {
+  const int layers = (int)lams.size();
+  std::vector<int> offs;
+  std::vector<double> w;
+  lattice_edge_order(kd, {nx, ny}, offs, w);
+  const int k = (int)kd.edges.size();
+  LatticeGeom g;
+  g.ndim = 2;
+  g.k = k;
+  g.D = layers;
+  g.H = nx;
+  g.W = ny;
+  for (int i = 0; i < k; ++i) {
+    g.dz.push_back(0);
+    g.dy.push_back(offs[i * 2]);
+    g.dx.push_back(offs[i * 2 + 1]);
+  }
+  std::vector<int64_t> pwf((size_t)layers * (k / 2));
+  for (int l = 0; l < layers; ++l)
+    for (int e = 0; e < k / 2; ++e) {
+      const int64_t p = tv_pwf(b, lams[l], w[e]);
+      if (p > ((int64_t)1 << 56) / k)
+        fail(LF_ERR_OVERFLOW, "lambda too small for the 64-bit fixed point of the TV network");
+      pwf[(size_t)l * (k / 2) + e] = p;
+    }
+  std::unique_ptr<MinCutEngine> eng = make_mincut_engine(g, 8);
+  eng->set_tv_network(b.uf.data(), pwf.data(), layers);
+  eng->pf_run_layers(tun, layers);
+  r_out.resize((size_t)layers * nx * ny);
+  hipStream_t s = eng->stream();
+  int64_t* d = nullptr;
+  LF_HIP(hipMallocAsync((void**)&d, sizeof(int64_t) * r_out.size(), s));
+  eng->pf_values_device(d, s);
+  LF_HIP(hipMemcpyAsync(r_out.data(), d, sizeof(int64_t) * r_out.size(), hipMemcpyDeviceToHost, s));
+  LF_HIP(hipFreeAsync(d, s));
+  LF_HIP(hipStreamSynchronize(s));
+  if (stats) *stats = eng->stats;
+}
+
+static const KernelDef& tv_kernel(const char* kernel) {
+  if (!kernel) fail(LF_ERR_BAD_ARG, "null argument");
+  const KernelDef* kd = find_kernel(kernel);
+  if (!kd) fail(LF_ERR_BAD_NAME, std::string("Bad Kernel or Solver Name: ") + kernel);
+  if (kd->ndim != 2 || !kd->geocut)
+    fail(LF_ERR_BAD_ARG, "Kernel is not valid for GeoCuts or TV Minimization (2-D geocut kernels only).");
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) fail(LF_ERR_DEVICE, "no HIP device available");
+  return *kd;
+}
+
 int lf_tv_single(const char* kernel, int64_t nx, int64_t ny, const double* f, double lambda, double* R,
                  lf_stats* stats) {
   return guarded([&] {
-    if (!kernel || !f || !R) fail(LF_ERR_BAD_ARG, "null argument");
-    const KernelDef* kd = find_kernel(kernel);
-    if (!kd) fail(LF_ERR_BAD_NAME, std::string("Bad Kernel or Solver Name: ") + kernel);
-    if (kd->ndim != 2 || !kd->geocut)
-      fail(LF_ERR_BAD_ARG, "Kernel is not valid for GeoCuts or TV Minimization (2-D geocut kernels only).");
+    const KernelDef& kd = tv_kernel(kernel);
+    if (!f || !R) fail(LF_ERR_BAD_ARG, "null argument");
     if (nx <= 0 || ny <= 0) fail(LF_ERR_BAD_ARG, "empty image");
-    const int64_t shape[2] = {nx, ny};
-    lf_handle* h = nullptr;
-    int rc = lf_create(&h, "LatticeLevelReductions", kernel, 2, shape, 8);
-    if (rc) fail(rc, g_last_error);
-    std::unique_ptr<lf_handle> hold(h);
-    TvSetup t = tv_setup(nx, ny, f, lambda, h->offs, h->geocut, h->k);
-    h->eng->set_base_network(t.red.data(), t.alpha.data());
     const auto t0 = std::chrono::steady_clock::now();
-    h->eng->pf_run(h->tuning);
-    std::vector<int64_t> r((size_t)(nx * ny));
-    hipStream_t s = h->eng->stream();
-    int64_t* d = nullptr;
-    LF_HIP(hipMallocAsync((void**)&d, sizeof(int64_t) * r.size(), s));
-    h->eng->pf_values_device(d, s);
-    LF_HIP(hipMemcpyAsync(r.data(), d, sizeof(int64_t) * r.size(), hipMemcpyDeviceToHost, s));
-    LF_HIP(hipFreeAsync(d, s));
-    LF_HIP(hipStreamSynchronize(s));
-    tv_output(t, r.data(), nx * ny, R);
-    if (stats) {
-      *stats = h->eng->stats;
-      stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const int64_t n = nx * ny;
+    TvBase b = tv_base(n, f);
+    std::vector<int64_t> r;
+    tv_solve_layers(kd, nx, ny, b, {lambda}, lf_tuning{}, r, stats);
+    tv_output_single(b, r.data(), n, R);
+    if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int lf_tv_path(const char* kernel, int64_t nx, int64_t ny, const double* f, int64_t n_lambda,
+               const double* lambdas, double* R, lf_stats* stats) {
+  return guarded([&] {
+    const KernelDef& kd = tv_kernel(kernel);
+    if (!f || !R || (n_lambda > 0 && !lambdas)) fail(LF_ERR_BAD_ARG, "null argument");
+    if (nx <= 0 || ny <= 0) fail(LF_ERR_BAD_ARG, "empty image");
+    if (n_lambda <= 0) return;  // the reference returns an empty path (tv_solver.hpp:664-667)
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t n = nx * ny;
+    // negative values count as 0 (tv_solver.hpp:669-674)
+    std::vector<double> lam(lambdas, lambdas + n_lambda);
+    for (double& l : lam) l = std::max(l, 0.0);
+    const double lmax = std::max(*std::max_element(lam.begin(), lam.end()), 1e-8);
+    TvBase b = tv_base(n, f);
+    // each positive lambda is an exact single-lambda problem; lambda = 0 is the
+    // fully merged solution (constant M)
+    std::vector<double> solve;
+    std::vector<int64_t> which(n_lambda, -1);
+    for (int64_t l = 0; l < n_lambda; ++l)
+      if (lam[l] > 0) {
+        which[l] = (int64_t)solve.size();
+        solve.push_back(lam[l]);
+      }
+    std::vector<int64_t> r;
+    if (!solve.empty()) tv_solve_layers(kd, nx, ny, b, solve, lf_tuning{}, r, stats);
+    std::vector<double> rp((size_t)n);
+    for (int64_t l = 0; l < n_lambda; ++l) {
+      double* out = R + l * n;
+      if (which[l] < 0) {
+        for (int64_t i = 0; i < n; ++i) out[i] = b.M;
+        continue;
+      }
+      // path value at s = lambda / lambda_max: r_path(s) = s r(lambda) by
+      // homogeneity of the TV problem (TVRegPathSegment::get_r_AtLambda,
+      // tv_regpath_node.hpp:146-150, is linear in s over each segment)
+      const double sc = lam[l] / lmax;
+      const int64_t* rl = r.data() + which[l] * n;
+      for (int64_t i = 0; i < n; ++i) rp[i] = sc * (double)rl[i];
+      tv_output_path(b, rp.data(), n, out);
     }
+    if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   });
 }
 

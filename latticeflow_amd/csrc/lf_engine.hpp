@@ -52,6 +52,12 @@ class MinCutEngine {
   // Parametric divide and conquer over the accumulated network
   // (ParametricFlowSolver::run, pf_solver.hpp:33-208): final r per node.
   virtual void pf_run(const lf_tuning& tuning) = 0;
+  // The same over a stack of `layers` equal lattices along the slowest axis
+  // (no arcs between layers): one initial region per layer.
+  virtual void pf_run_layers(const lf_tuning& tuning, int layers) = 0;
+  // TV network of a stack of images: uf[H*W] per-pixel function values, pwf[layers][k/2]
+  // per-direction pairwise values (tv_solver.hpp:71-112).
+  virtual void set_tv_network(const int64_t* uf, const int64_t* pwf, int layers) = 0;
   virtual void pf_values_device(int64_t* d_out, hipStream_t s) = 0;
   virtual void pf_cut_device(int32_t* d_out, hipStream_t s) = 0;  // r < 0
   // Replace the base network (host arrays: red[N], alpha[N*k] node-major, lattice order).

@@ -39,6 +39,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -920,14 +922,6 @@ __device__ inline double i128_to_double(unsigned long long hi, unsigned long lon
   return neg ? -d : d;
 }
 
-__global__ __launch_bounds__(NTHREADS) void k_pf_init(PFP f, const long long* __restrict__ red0) {
-  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
-  if (v >= f.N) return;
-  f.r[v] = red0[v];
-  f.reg[v] = 0;
-  if (v == 0) f.tab[0].state = 1;
-}
-
 __global__ __launch_bounds__(NTHREADS) void k_pf_clear(PFP f, int nreg) {
   const int k = blockIdx.x * NTHREADS + threadIdx.x;
   if (k >= nreg) return;
@@ -1102,6 +1096,39 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_cut(PFP f, int* __restrict__ ou
   out[v] = f.r[v] < 0 ? 1 : 0;
 }
 
+// TV network of a stack of `layers` images (one lambda per layer): reduction
+// 2 uf (addE1(0, uf)) and, on every in-lattice arc of direction i,
+// 2 pwf[layer][i mod k/2] (addE2(0, pwf, pwf, 0)) (tv_solver.hpp:71-112).
+template <int K>
+__global__ __launch_bounds__(NTHREADS) void k_tv_net(int D, int H, int W, long long N, DirTab dirs,
+                                                     const long long* __restrict__ uf,
+                                                     const long long* __restrict__ pwf,
+                                                     long long* __restrict__ red0, long long* __restrict__ alpha0) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= N) return;
+  const long long plane = (long long)H * W;
+  const int z = (int)(v / plane);
+  const long long q = v - (long long)z * plane;
+  const int y = (int)(q / W), x = (int)(q % W);
+  red0[v] = 2 * uf[q];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int ny = y + dirs.dy[i], nx = x + dirs.dx[i];
+    const bool in = ny >= 0 && ny < H && nx >= 0 && nx < W;
+    alpha0[(long long)i * N + v] = in ? 2 * pwf[(long long)z * (K / 2) + (i % (K / 2))] : 0;
+  }
+  (void)D;
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_pf_init_layers(PFP f, const long long* __restrict__ red0,
+                                                             long long plane, int layers) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  f.r[v] = red0[v];
+  f.reg[v] = (int)(v / plane);
+  if (v < layers) f.tab[v].state = 1;
+}
+
 inline unsigned blocks_for(long long n) { return (unsigned)((n + NTHREADS - 1) / NTHREADS); }
 
 // ---- engine -----------------------------------------------------------------------
@@ -1228,10 +1255,14 @@ class Engine final : public MinCutEngine {
     long long cycles = 0;
     for (;;) {
       const auto tb = clk::now();
+      const long long bl0 = stats.bfs_launches;
       const long long active = global_relabel(p, pushed);
       stats.bfs_seconds += std::chrono::duration<double>(clk::now() - tb).count();
       ++stats.cycles;
       ++cycles;
+      if (trace_)
+        std::fprintf(stderr, "[lf] cycle %lld active %lld bfs_launches %lld tiles_pushed %llu\n", cycles, active,
+                     stats.bfs_launches - bl0, (unsigned long long)read_tiles());
       if (active == 0) break;
       if (tun.max_cycles > 0 && cycles > tun.max_cycles)
         fail(LF_ERR_STATE, "max_cycles reached before the flow became maximum");
@@ -1290,7 +1321,26 @@ class Engine final : public MinCutEngine {
   }
 
   // ---- parametric divide and conquer (LatticeLevelReductions / TV) ----------
-  void pf_run(const lf_tuning& tun) override {
+  void pf_run(const lf_tuning& tun) override { pf_run_layers(tun, 1); }
+
+  void set_tv_network(const int64_t* uf, const int64_t* pwf, int layers) override {
+    const long long plane = N_ / layers;
+    long long *duf = nullptr, *dpwf = nullptr;
+    LF_HIP(hipMallocAsync((void**)&duf, sizeof(long long) * plane, s_));
+    LF_HIP(hipMallocAsync((void**)&dpwf, sizeof(long long) * layers * (K / 2), s_));
+    LF_HIP(hipMemcpyAsync(duf, uf, sizeof(long long) * plane, hipMemcpyHostToDevice, s_));
+    LF_HIP(hipMemcpyAsync(dpwf, pwf, sizeof(long long) * layers * (K / 2), hipMemcpyHostToDevice, s_));
+    hipLaunchKernelGGL((k_tv_net<K>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p_.D, p_.H, p_.W, N_, p_.dirs,
+                       duf, dpwf, red0_, alpha0_);
+    LF_HIP(hipGetLastError());
+    LF_HIP(hipFreeAsync(duf, s_));
+    LF_HIP(hipFreeAsync(dpwf, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    solved = false;
+    pf_solved = false;
+  }
+
+  void pf_run_layers(const lf_tuning& tun, int layers) override {
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     begin_stats();
@@ -1301,8 +1351,9 @@ class Engine final : public MinCutEngine {
     // one region holding every node
     hipLaunchKernelGGL((k_fill<char>), dim3(blocks_for((long long)sizeof(RegionStat) * N_)), dim3(NTHREADS), 0, s_,
                        reinterpret_cast<char*>(pf.tab), (char)0, (long long)sizeof(RegionStat) * N_);
-    hipLaunchKernelGGL((k_pf_init), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, red0_);
-    hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(NTHREADS), 0, s_, pf.nreg, 1, 1LL);
+    // one region per layer
+    hipLaunchKernelGGL((k_pf_init_layers), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, red0_, N_ / layers, layers);
+    hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(NTHREADS), 0, s_, pf.nreg, layers, 1LL);
     long long levels = 0, sections = 0;
     for (;;) {
       // region statistics and thresholds (setToMeanReductionPoint)
@@ -1392,6 +1443,11 @@ class Engine final : public MinCutEngine {
   unsigned long long* tiles_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 24); }
   void zero_counter() {
     hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, counter_ptr(), 0ull, 1LL);
+  }
+  unsigned long long read_tiles() {
+    LF_HIP(hipMemcpyAsync(host_small_, tiles_ptr(), sizeof(unsigned long long), hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    return *reinterpret_cast<unsigned long long*>(host_small_);
   }
   int read_int(const int* d) {
     LF_HIP(hipMemcpyAsync(host_small_, d, sizeof(int), hipMemcpyDeviceToHost, s_));
@@ -1497,6 +1553,7 @@ class Engine final : public MinCutEngine {
   void* host_small_ = nullptr;
   hipEvent_t ev_[2] = {nullptr, nullptr};
   unsigned long long max_red_ = 0, max_alpha_ = 0;
+  const bool trace_ = std::getenv("LF_TRACE") != nullptr;
   long long* pf_r_ = nullptr;
   int* pf_reg_ = nullptr;
   RegionStat* pf_tab_ = nullptr;

@@ -256,3 +256,27 @@ def calculate2dTV(X, lam, kernel="Star2d_16", return_stats=False):
     R, st = _tv_single(X, lam, kernel)
     R = R * lam
     return (R, st) if return_stats else R
+
+
+def _tv_path(X, lambdas, kernel):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    if X.ndim != 2:
+        raise ValueError("calculate2dTVPath needs a 2-d array")
+    lam = np.ascontiguousarray(lambdas, dtype=np.float64).reshape(-1)
+    nx, ny = X.shape
+    R = np.empty((lam.size, nx, ny), dtype=np.float64)
+    st = _lib.lf_stats()
+    check(lib().lf_tv_path(kernel.encode(), nx, ny, X.ctypes.data, lam.size, lam.ctypes.data,
+                           R.ctypes.data, ctypes.byref(st)))
+    return R, st.as_dict()
+
+
+def calculate2dTVPath(X, lambdas, kernel="Star2d_16", return_stats=False):
+    """pyx:457-484: the exact regularization path of X at ``lambdas``.  As in the
+    reference's Python layer, every slice is divided by the last lambda of the
+    list (its loop index is stale there)."""
+    R, st = _tv_path(X, lambdas, kernel)
+    lam = np.asarray(lambdas, dtype=np.float64).reshape(-1)
+    if lam.size:
+        R = R / lam[-1]
+    return (R, st) if return_stats else R

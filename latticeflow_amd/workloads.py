@@ -169,3 +169,21 @@ def n_edges(kernel_offsets, shape):
             m *= max(0, shape[a] - abs(int(d)))
         tot += m
     return tot
+
+
+def tv_voronoi(seed=5, shape=(1024, 1024), cells=256, noise=0.1):
+    """Piecewise-constant image: ``cells`` random Voronoi cells with values U[0,1],
+    plus N(0, noise) (the TV regularization-path workload)."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    H, W = shape
+    pts = rng.random((cells, 2)) * np.array([H, W])
+    vals = rng.random(cells)
+    yy, xx = np.mgrid[0:H, 0:W]
+    _, idx = cKDTree(pts).query(np.stack([yy.ravel() + 0.5, xx.ravel() + 0.5], axis=1))
+    img = vals[idx].reshape(shape)
+    return img + rng.normal(0, noise, shape)
+
+
+def tv_lambdas(count=32, lo=0.01, hi=2.0):
+    return np.exp(np.linspace(np.log(lo), np.log(hi), count))

@@ -920,3 +920,72 @@ int orc_tv_single(int64_t nx, int64_t ny, int k, const int32_t *offs, const doub
   free(red); free(alpha); free(r);
   return 0;
 }
+
+/* Regularization path at given lambdas (calculate2dTV with a lambda vector,
+ * tv_solver.hpp:659-681, output getRegularizationPath :181-244).  The
+ * reference builds the path by events; each segment's value is linear in
+ * s = lambda / lambda_max (get_r_AtLambda, tv_regpath_node.hpp:146-150), and by
+ * homogeneity of the TV problem r_path(s) = s r(lambda) with r(lambda) the
+ * single-lambda solution, so the path is restated from single-lambda divide
+ * and conquer solves.  R: n_lambda * nx * ny. */
+int orc_tv_path(int64_t nx, int64_t ny, int k, const int32_t *offs, const double *w,
+                const double *f, int64_t n_lambda, const double *lambdas, double *R) {
+  const int64_t n = nx * ny;
+  if (n <= 0) return -1;
+  if (n_lambda <= 0) return 0;
+  double v_min = f[0], v_max = f[0], v_sum = f[0];
+  for (int64_t i = 1; i < n; ++i) {
+    double v = f[i];
+    v_min = v_min < v ? v_min : v;
+    v_max = v_max > v ? v_max : v;
+    v_sum += v;
+  }
+  const double M = v_sum / (double)n;
+  v_min -= M;
+  v_max -= M;
+  const double S = 2 * (v_max > -v_min ? v_max : -v_min);
+  double lmax = 0;
+  for (int64_t l = 0; l < n_lambda; ++l) {
+    double lm = lambdas[l] < 0 ? 0 : lambdas[l];
+    if (lm > lmax) lmax = lm;
+  }
+  if (lmax < 1e-8) lmax = 1e-8;
+  int64_t shape[2] = {nx, ny};
+  ulat_t L;
+  if (ulat_init(&L, 2, shape, k, offs)) return -1;
+  int64_t *red = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+  int64_t *alpha = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n * k));
+  int64_t *r = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+  for (int64_t l = 0; l < n_lambda; ++l) {
+    double lam = lambdas[l] < 0 ? 0 : lambdas[l];
+    double *out = R + l * n;
+    if (lam <= 0) {
+      for (int64_t i = 0; i < n; ++i) out[i] = M;
+      continue;
+    }
+    for (int64_t i = 0; i < n; ++i) red[i] = 2 * tv_to_fv(0.5 + (f[i] - M) / S);
+    memset(alpha, 0, sizeof(int64_t) * (size_t)(n * k));
+    const double gsv = 0.5 / (S * (lam > 1e-8 ? lam : 1e-8));
+    for (int64_t v = 0; v < n; ++v)
+      for (int e = 0; e < k / 2; ++e) {
+        int64_t u = ulat_nbr(&L, v, e);
+        if (u < 0) continue;
+        int64_t pwf = tv_to_fv((gsv * w[e]) / S);
+        alpha[v * k + e] += 2 * pwf;
+        alpha[u * k + rev_dir(k, e)] += 2 * pwf;
+      }
+    int rc = orc_pf_solve(2, shape, k, offs, red, alpha, r, NULL, NULL);
+    if (rc) { free(red); free(alpha); free(r); return rc; }
+    const double sc = lam / lmax;
+    double sum = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      double v = ((sc * (double)r[i]) / (double)((int64_t)1 << TV_FBITS) - 0.5) * S + M;
+      out[i] = v;
+      sum += v;
+    }
+    const double mean = sum / (double)n;
+    for (int64_t i = 0; i < n; ++i) out[i] = (out[i] - mean) * S + M;
+  }
+  free(red); free(alpha); free(r);
+  return 0;
+}

@@ -164,8 +164,10 @@ def lib():
         f64p = ctypes.POINTER(ctypes.c_double)
         L.orc_tv_single.argtypes = [ctypes.c_int64, ctypes.c_int64, ci, i32p, f64p, f64p,
                                     ctypes.c_double, f64p, i64p]
+        L.orc_tv_path.argtypes = [ctypes.c_int64, ctypes.c_int64, ci, i32p, f64p, f64p, ctypes.c_int64,
+                                  f64p, f64p]
         for f in (L.orc_reparam, L.orc_prflow_solve, L.orc_dinic, L.orc_cuts, L.orc_pf_solve,
-                  L.orc_tv_single):
+                  L.orc_tv_single, L.orc_tv_path):
             f.restype = ci
         _LIB = L
     return _LIB
@@ -284,6 +286,21 @@ def tv_single(f, lam, kernel="Full2d_8"):
                              _p(R, ctypes.c_double), _p(st, ctypes.c_int64))
     assert rc == 0, rc
     return R.reshape(nx, ny), dict(sections=int(st[0]), n_levels=int(st[1]))
+
+
+def tv_path(f, lambdas, kernel="Full2d_8"):
+    """Regularization-path output (TVSolver::getRegularizationPath conventions)."""
+    f = np.ascontiguousarray(f, np.float64)
+    nx, ny = f.shape
+    offs, w = edge_order(kernel, (nx, ny))
+    offs = np.ascontiguousarray(offs, np.int32)
+    w = np.ascontiguousarray(w, np.float64)
+    lam = np.ascontiguousarray(lambdas, np.float64).reshape(-1)
+    R = np.zeros((lam.size, nx, ny), np.float64)
+    rc = lib().orc_tv_path(nx, ny, int(offs.shape[0]), _p(offs, ctypes.c_int32), _p(w, ctypes.c_double),
+                           _p(f, ctypes.c_double), lam.size, _p(lam, ctypes.c_double), _p(R, ctypes.c_double))
+    assert rc == 0, rc
+    return R
 
 
 def tv1d_condat(y, lam):

@@ -67,3 +67,27 @@ def test_calculate2dTV_python_layer(oracle):
     R = calculate2dTV(f, 3.0)
     Rr, _ = oracle.tv_single(f, 3.0, "Star2d_16")
     assert np.abs(R - 3.0 * Rr).max() <= 1e-9 * np.abs(3.0 * Rr).max()
+
+
+@pytest.mark.parametrize("kernel", ["Full2d_8", "Star2d_16"])
+def test_tv_path_vs_oracle(oracle, kernel):
+    from latticeflow_amd.pylatticeflow import _tv_path
+    from latticeflow_amd import workloads as W
+    f = W.tv_voronoi(seed=11, shape=(48, 40), cells=12)
+    lams = [0.02, 0.3, 1.0, 2.0, 0.0, 5.0]
+    R, st = _tv_path(f, lams, kernel)
+    Rr = oracle.tv_path(f, lams, kernel)
+    rel = np.abs(R - Rr).max() / np.abs(Rr).max()
+    print(kernel, rel, st, flush=True)
+    assert rel <= 1e-9
+
+
+def test_calculate2dTVPath_python_layer(oracle):
+    from latticeflow_amd import calculate2dTVPath
+    rng = np.random.default_rng(5)
+    f = rng.random((16, 18))
+    lams = np.array([4.0, 1.0, 2.0])
+    R = calculate2dTVPath(f, lams)
+    Rr = oracle.tv_path(f, lams, "Star2d_16") / lams[-1]
+    assert np.abs(R - Rr).max() <= 1e-9 * np.abs(Rr).max()
+    assert calculate2dTVPath(f, []).shape == (0, 16, 18)
