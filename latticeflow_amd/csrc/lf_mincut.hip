@@ -1249,39 +1249,61 @@ class Engine final : public MinCutEngine {
   void solve_state(const lf_tuning& tun) {
     using clk = std::chrono::steady_clock;
     KP<T> p = params();
-    p.iters = tun.local_iters > 0 ? tun.local_iters : 4;
-    const int R = tun.rounds_per_relabel > 0 ? tun.rounds_per_relabel : 4;
+    // Fixed tuning when given; otherwise the number of push rounds between two
+    // exact relabels follows the measured cost ratio of a relabel and a round
+    // (a round over few active tiles is cheap, a relabel always covers the
+    // lattice), pushing stops early once no tile is active, and the local
+    // iterations per round grow when few nodes are active.
+    const bool fixed_r = tun.rounds_per_relabel > 0;
+    int R = fixed_r ? tun.rounds_per_relabel : 4;
     bool pushed = false;
     long long cycles = 0;
     for (;;) {
       const auto tb = clk::now();
       const long long bl0 = stats.bfs_launches;
       const long long active = global_relabel(p, pushed);
-      stats.bfs_seconds += std::chrono::duration<double>(clk::now() - tb).count();
+      const double t_relabel = std::chrono::duration<double>(clk::now() - tb).count();
+      stats.bfs_seconds += t_relabel;
       ++stats.cycles;
       ++cycles;
       if (trace_)
-        std::fprintf(stderr, "[lf] cycle %lld active %lld bfs_launches %lld tiles_pushed %llu\n", cycles, active,
-                     stats.bfs_launches - bl0, (unsigned long long)read_tiles());
+        std::fprintf(stderr, "[lf] cycle %lld active %lld bfs_launches %lld tiles_pushed %llu R %d\n", cycles, active,
+                     stats.bfs_launches - bl0, (unsigned long long)read_tiles(), R);
       if (active == 0) break;
       if (tun.max_cycles > 0 && cycles > tun.max_cycles)
         fail(LF_ERR_STATE, "max_cycles reached before the flow became maximum");
-      LF_HIP(hipEventRecord(ev_[0], s_));
-      for (int r = 0; r < R; ++r) {
-        KP<T> q = p;
-        q.flag_in = flags_[cur_];
-        q.flag_out = flags_[cur_ ^ 1];
-        hipLaunchKernelGGL((k_push<K, G, T>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q);
-        cur_ ^= 1;
-        ++stats.rounds;
+      p.iters = tun.local_iters > 0 ? tun.local_iters : (active * 64 < N_ ? 16 : 4);
+      const int chunk = fixed_r ? R : 4;
+      int done_rounds = 0;
+      double t_push = 0;
+      while (done_rounds < R) {
+        const unsigned long long tiles0 = fixed_r ? 0 : read_tiles();
+        LF_HIP(hipEventRecord(ev_[0], s_));
+        const int n = std::min(chunk, R - done_rounds);
+        for (int r = 0; r < n; ++r) {
+          KP<T> q = p;
+          q.flag_in = flags_[cur_];
+          q.flag_out = flags_[cur_ ^ 1];
+          hipLaunchKernelGGL((k_push<K, G, T>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q);
+          cur_ ^= 1;
+          ++stats.rounds;
+        }
+        LF_HIP(hipGetLastError());
+        LF_HIP(hipEventRecord(ev_[1], s_));
+        pushed = true;
+        LF_HIP(hipEventSynchronize(ev_[1]));
+        float ms = 0;
+        LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        t_push += ms * 1e-3;
+        done_rounds += n;
+        if (!fixed_r && read_tiles() == tiles0) break;  // no tile had work
       }
-      LF_HIP(hipGetLastError());
-      LF_HIP(hipEventRecord(ev_[1], s_));
-      pushed = true;
-      LF_HIP(hipEventSynchronize(ev_[1]));
-      float ms = 0;
-      LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-      stats.push_seconds += ms * 1e-3;
+      stats.push_seconds += t_push;
+      if (!fixed_r) {
+        const double per_round = t_push / std::max(done_rounds, 1);
+        const double want = per_round > 0 ? t_relabel / per_round : 4.0;
+        R = (int)std::max(4.0, std::min(256.0, want));
+      }
     }
     const auto tf = clk::now();
     hipLaunchKernelGGL((k_flood_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, reach_);
