@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args()
+    if a.workload == "tvpath":
+        return bench_tvpath(a)
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the solver has no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
@@ -144,6 +146,54 @@ def main():
                    "flow": flow, "parity_vs_cpu": parity,
                    "cycles": st["cycles"], "rounds": st["rounds"], "bfs_launches": st["bfs_launches"]},
         "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_tvpath(a):
+    """Exact TV regularization path: 1024x1024 Voronoi image, Full2d_8, 32 lambdas
+    log-spaced in [0.01, 2], all lambdas solved together on the device (plain
+    divide and conquer, threshold ceil(mean))."""
+    import numpy as np
+    import torch  # noqa: F401  (initialises the device like the main bench)
+    from latticeflow_amd import workloads as W
+    from latticeflow_amd.pylatticeflow import _tv_path
+    shape = tuple(int(x) for x in a.shape.split("x")) if a.shape else (1024, 1024)
+    f = W.tv_voronoi(seed=5, shape=shape)
+    lams = W.tv_lambdas()
+    for _ in range(a.warmup):
+        _tv_path(f, lams, "Full2d_8")
+    times = []
+    for _ in range(max(a.steps, 1)):
+        t0 = time.perf_counter()
+        R, st = _tv_path(f, lams, "Full2d_8")
+        times.append(time.perf_counter() - t0)
+    dt = sum(times) / len(times)
+    cpu = None
+    parity = None
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        pick = [16, 31]
+        tc = time.perf_counter()
+        Rr = O.tv_path(f, lams[pick], "Full2d_8")
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": round(tcpu / len(pick) * len(lams), 2), "unit": "s (32-lambda path, extrapolated)",
+               "cores": 1, "kind": "port",
+               "sample": "lambdas %s on the full %s image with the oracle's divide and conquer, %.2f s for %d "
+                         "lambdas, scaled to 32" % ([round(float(lams[i]), 4) for i in pick],
+                                                     "x".join(map(str, shape)), tcpu, len(pick))}
+        rel = float(np.abs(R[pick] - Rr).max() / np.abs(Rr).max())
+        parity = {"max_rel_err": rel, "within_1e-9": rel <= 1e-9}
+    out = {
+        "metric": "TV-path wall time (exact, 32 lambdas), plain divide and conquer",
+        "value": round(dt, 3), "unit": "s", "n_gpus": 1, "steps": len(times), "warmup": a.warmup,
+        "ms_per_step": round(dt * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+        "config": {"workload": "tvpath %s Full2d_8 Voronoi(256 cells)+N(0,0.1), 32 lambdas log[0.01,2]"
+                   % "x".join(map(str, shape)), "levels_max": st["reserved"], "cycles": st["cycles"],
+                   "rounds": st["rounds"], "bfs_launches": st["bfs_launches"], "parity_vs_cpu": parity},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
