@@ -40,7 +40,7 @@ typedef struct {
   int32_t rounds_per_relabel; /* push/relabel rounds between exact global relabels */
   int32_t local_iters;        /* push/relabel iterations per tile per round, in LDS */
   int32_t max_cycles;         /* cap on global-relabel cycles (0 = none) */
-  int32_t reserved;
+  int32_t tile_shape;         /* push-tile shape: 0 = default, 1..3 = the shapes of lf_autotune */
 } lf_tuning;
 
 /* Solver statistics of the last lf_run. */
@@ -109,6 +109,13 @@ int lf_reset_energies(lf_handle *h);
 /* run() (interface.hpp:113, direct_interface.hpp:49-58).  Solves from the
  * accumulated energies; tuning may be NULL (autotuned defaults). */
 int lf_set_tuning(lf_handle *h, const lf_tuning *tuning);
+int lf_get_tuning(const lf_handle *h, lf_tuning *tuning);
+/* Autotuner (no reference counterpart; the reference fixes its policies at
+ * compile time, kernels/optimization_policies.hpp:4-41): solves the current
+ * network `reps` times under each candidate tile shape and relabel schedule,
+ * keeps the fastest as the handle's tuning and returns it in *best.  Every
+ * candidate yields the same flow and cuts. */
+int lf_autotune(lf_handle *h, int reps, lf_tuning *best);
 int lf_run(lf_handle *h);
 /* runTime() (interface.hpp:115): wall seconds of the last lf_run. */
 int lf_run_time(const lf_handle *h, double *seconds);

@@ -38,7 +38,10 @@ class LatticeFlowError(RuntimeError):
 
 class lf_tuning(ctypes.Structure):
     _fields_ = [("rounds_per_relabel", ctypes.c_int32), ("local_iters", ctypes.c_int32),
-                ("max_cycles", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("max_cycles", ctypes.c_int32), ("tile_shape", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class lf_stats(ctypes.Structure):
@@ -78,6 +81,8 @@ _SIG = {
     "lf_add_energy_potentials_device": (_I, [_P, _P, _P, _P]),
     "lf_reset_energies": (_I, [_P]),
     "lf_set_tuning": (_I, [_P, ctypes.POINTER(lf_tuning)]),
+    "lf_get_tuning": (_I, [_P, ctypes.POINTER(lf_tuning)]),
+    "lf_autotune": (_I, [_P, _I, ctypes.POINTER(lf_tuning)]),
     "lf_run": (_I, [_P]),
     "lf_run_time": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "lf_get_stats": (_I, [_P, ctypes.POINTER(lf_stats)]),

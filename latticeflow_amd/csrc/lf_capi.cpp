@@ -247,6 +247,47 @@ int lf_set_tuning(lf_handle* h, const lf_tuning* t) {
   });
 }
 
+int lf_get_tuning(const lf_handle* h, lf_tuning* t) {
+  return guarded([&] {
+    if (!h || !t) fail(LF_ERR_BAD_ARG, "null argument");
+    *t = h->tuning;
+  });
+}
+
+int lf_autotune(lf_handle* h, int reps, lf_tuning* best) {
+  return guarded([&] {
+    if (!h) fail(LF_ERR_BAD_ARG, "null argument");
+    if (reps < 1) reps = 1;
+    // candidates: the adaptive schedule and two fixed ones, on every tile shape
+    const lf_tuning sched[] = {{0, 0, 0, 0}, {4, 4, 0, 0}, {8, 2, 0, 0}, {16, 1, 0, 0}};
+    lf_tuning best_t = h->tuning;
+    double best_s = 1e300;
+    for (int shape = 1; shape <= 3; ++shape)
+      for (const lf_tuning& c : sched) {
+        lf_tuning t = c;
+        t.tile_shape = shape;
+        // one untimed warm-up, then the mean of `reps` timed solves (the
+        // schedule of a push-relabel run is not deterministic, so a candidate
+        // that is only sometimes fast must not win on its best run)
+        if (h->levels) h->eng->pf_run(t);
+        else h->eng->run(t);
+        double tot = 0;
+        for (int r = 0; r < reps; ++r) {
+          const auto t0 = std::chrono::steady_clock::now();
+          if (h->levels) h->eng->pf_run(t);
+          else h->eng->run(t);
+          tot += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        if (tot < best_s) {
+          best_s = tot;
+          best_t = t;
+        }
+      }
+    h->tuning = best_t;
+    if (best) *best = best_t;
+  });
+}
+
 int lf_run(lf_handle* h) {
   return guarded([&] {
     if (!h) fail(LF_ERR_BAD_ARG, "null argument");

@@ -212,11 +212,12 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   constexpr int NT = G::NT;
   __shared__ T s_res[K * NT];
   __shared__ T s_ex[NT];
-  __shared__ T s_in[NT];
   __shared__ int s_h[G::NH];
   __shared__ unsigned short s_list[NT];
+  __shared__ unsigned int s_dirty[NT];  // bit i: residual plane i changed; bit 31: excess or label
   __shared__ int s_cnt[2];
   __shared__ int s_flag;
+  static_assert(K <= 31, "dirty mask holds 31 planes");
 
   const int tile = tile_of_block(p.nblocks);
   if (tile >= p.ntiles) return;
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
       for (int i = 0; i < K; ++i) s_res[i * NT + t] = 0;
       s_ex[t] = 0;
     }
-    s_in[t] = 0;
+    s_dirty[t] = 0;
   }
   for (int q = threadIdx.x; q < G::NH; q += NTHREADS) {
     const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
@@ -286,25 +287,19 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         const T a = gtake(slot);
         s_ex[t] += a;
         s_res[i * NT + t] += a;
+        s_dirty[t] |= (1u << i) | 0x80000000u;
       }
     }
   }
   __syncthreads();
 
   for (int it = 0; it < p.iters; ++it) {
-    // fold last iteration's inflow into the excess and list the active nodes
-    // (ex > 0, label < INF) with one LDS atomic per wave
+    // list the active nodes (ex > 0, label < INF) with one LDS atomic per wave
     int* cnt = &s_cnt[it & 1];
 #pragma unroll
     for (int j = 0; j < G::NPT; ++j) {
       const int t = threadIdx.x + j * NTHREADS;
-      const T a = s_in[t];
-      T e = s_ex[t];
-      if (a != 0) {
-        e += a;
-        s_ex[t] = e;
-        s_in[t] = 0;
-      }
+      const T e = s_ex[t];
       int z, y, x;
       tile_coords<G>(t, z, y, x);
       const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
@@ -328,7 +323,11 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
       tile_coords<G>(t, z, y, x);
       const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
       const int hv = s_h[hi];
-      T e = s_ex[t];
+      // excess moves only through LDS atomics in this phase: in-tile receivers
+      // add, the owner subtracts what it pushed
+      const T e0 = s_ex[t];
+      T e = e0;
+      unsigned int dirty = 0;
 #pragma unroll
       for (int i = 0; i < K; ++i) {
         const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
@@ -339,11 +338,13 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
             const T d = r < e ? r : e;
             s_res[i * NT + t] = r - d;
             e -= d;
+            dirty |= 1u << i;
             const int lz = z + dz, ly = y + dy, lx = x + dx;
             if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) {
               const int tn = t + (dz * G::TH + dy) * G::TW + dx;
               s_res[((i + K / 2) % K) * NT + tn] += d;
-              lds_add(&s_in[tn], d);
+              lds_add(&s_ex[tn], d);
+              atomicOr(&s_dirty[tn], (1u << ((i + K / 2) % K)) | 0x80000000u);
             } else {
               const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
               gadd(&p.mbox[(long long)i * N + g], d);
@@ -352,7 +353,10 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
           }
         }
       }
-      s_ex[t] = e;
+      if (e != e0) {
+        lds_add(&s_ex[t], e - e0);
+        atomicOr(&s_dirty[t], dirty | 0x80000000u);
+      }
     }
     __syncthreads();
 
@@ -373,12 +377,15 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
           m = hn < m ? hn : m;
         }
       }
-      if (m >= hv) s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
+      if (m >= hv) {
+        s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
+        atomicOr(&s_dirty[t], 0x80000000u);
+      }
     }
     __syncthreads();
   }
 
-  // write the tile back (folding any inflow of the last iteration)
+  // write back what changed (residual planes, excess and label of touched nodes)
   bool act = false;
 #pragma unroll
   for (int j = 0; j < G::NPT; ++j) {
@@ -386,13 +393,17 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
     int z, y, x;
     tile_coords<G>(t, z, y, x);
     if (z >= ld || y >= lh || x >= lw) continue;
-    const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
-#pragma unroll
-    for (int i = 0; i < K; ++i) p.res[i * N + g] = s_res[i * NT + t];
-    const T e = s_ex[t] + s_in[t];
+    const unsigned int dm = s_dirty[t];
+    const T e = s_ex[t];
     const int hv = s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)];
-    p.ex[g] = e;
-    p.h[g] = hv;
+    if (dm) {
+      const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+#pragma unroll
+      for (int i = 0; i < K; ++i)
+        if (dm & (1u << i)) p.res[i * N + g] = s_res[i * NT + t];
+      p.ex[g] = e;
+      p.h[g] = hv;
+    }
     if (e > 0 && hv < INF_H) act = true;
   }
   if (__syncthreads_or(act)) {
@@ -1129,24 +1140,51 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_init_layers(PFP f, const long l
   if (v < layers) f.tab[v].state = 1;
 }
 
+__global__ __launch_bounds__(NTHREADS) void k_max_label(const int* __restrict__ h, long long N, long long* part) {
+  long long m = 0;
+  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < N; v += (long long)gridDim.x * NTHREADS) {
+    const int x = h[v];
+    if (x < INF_H && x > m) m = x;
+  }
+  m = block_reduce<true>(m);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
 inline unsigned blocks_for(long long n) { return (unsigned)((n + NTHREADS - 1) / NTHREADS); }
 
 // ---- engine -----------------------------------------------------------------------
 
-template <int K, class G, class B, typename T, typename MT>
+struct TileGrid {
+  int ntx = 0, nty = 0, ntz = 0, ntiles = 0, nblocks = 0, nt = 0;
+};
+
+template <class G>
+TileGrid make_grid(const LatticeGeom& geo) {
+  TileGrid t;
+  t.ntx = (int)((geo.W + G::TW - 1) / G::TW);
+  t.nty = (int)((geo.H + G::TH - 1) / G::TH);
+  t.ntz = (int)((geo.D + G::TD - 1) / G::TD);
+  const long long n = (long long)t.ntx * t.nty * t.ntz;
+  if (n > (1LL << 30)) fail(LF_ERR_BAD_ARG, "lattice too large");
+  t.ntiles = (int)n;
+  t.nblocks = (int)(((n + 7) / 8) * 8);
+  t.nt = G::NT;
+  return t;
+}
+
+// G0..G2: the push-tile shapes the tuner chooses from (same halo radius).
+template <int K, class G0, class G1, class G2, class B, typename T, typename MT>
 class Engine final : public MinCutEngine {
  public:
   Engine(const LatticeGeom& geo) : g_(geo) {
     LF_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     N_ = geo.n();
     p_.D = (int)geo.D; p_.H = (int)geo.H; p_.W = (int)geo.W; p_.N = N_;
-    p_.ntx = (int)((geo.W + G::TW - 1) / G::TW);
-    p_.nty = (int)((geo.H + G::TH - 1) / G::TH);
-    p_.ntz = (int)((geo.D + G::TD - 1) / G::TD);
-    ntiles_ = (long long)p_.ntx * p_.nty * p_.ntz;
-    if (ntiles_ > (1LL << 30)) fail(LF_ERR_BAD_ARG, "lattice too large");
-    p_.ntiles = (int)ntiles_;
-    p_.nblocks = (int)(((ntiles_ + 7) / 8) * 8);
+    grid_[0] = make_grid<G0>(geo);
+    grid_[1] = make_grid<G1>(geo);
+    grid_[2] = make_grid<G2>(geo);
+    ntiles_ = std::max(grid_[0].ntiles, std::max(grid_[1].ntiles, grid_[2].ntiles));
+    set_grid(0);
     bp_.D = p_.D; bp_.H = p_.H; bp_.W = p_.W; bp_.N = N_;
     bp_.ntx = (int)((geo.W + B::TW - 1) / B::TW);
     bp_.nty = (int)((geo.H + B::TH - 1) / B::TH);
@@ -1191,6 +1229,23 @@ class Engine final : public MinCutEngine {
   }
 
   hipStream_t stream() const override { return s_; }
+
+  void set_grid(int v) {
+    variant_ = v;
+    p_.ntx = grid_[v].ntx;
+    p_.nty = grid_[v].nty;
+    p_.ntz = grid_[v].ntz;
+    p_.ntiles = grid_[v].ntiles;
+    p_.nblocks = grid_[v].nblocks;
+  }
+  template <class F>
+  void with_geo(F&& f) {
+    switch (variant_) {
+      case 1: f(G1{}); break;
+      case 2: f(G2{}); break;
+      default: f(G0{}); break;
+    }
+  }
 
   void reset_energies() override {
     LF_HIP(hipMemsetAsync(red0_, 0, sizeof(long long) * N_, s_));
@@ -1256,20 +1311,39 @@ class Engine final : public MinCutEngine {
     // iterations per round grow when few nodes are active.
     const bool fixed_r = tun.rounds_per_relabel > 0;
     int R = fixed_r ? tun.rounds_per_relabel : 4;
+    // a solve never switches tilings midway: the mailbox is empty here
+    if (tun.tile_shape >= 1 && tun.tile_shape <= 3) set_grid(tun.tile_shape - 1);
+    else set_grid(0);
+    p = params();
     bool pushed = false;
     long long cycles = 0;
+    // Intermediate relabels stop after kRelabelCap launches; the stopping
+    // decision is always taken on an exact (uncapped) relabel.
+    constexpr long long kRelabelCap = 16;
+    bool need_exact = false;
     for (;;) {
       const auto tb = clk::now();
       const long long bl0 = stats.bfs_launches;
-      const long long active = global_relabel(p, pushed);
+      bool exact = false;
+      const long long active = global_relabel(p, pushed, need_exact ? 0 : kRelabelCap, &exact);
+      pushed = false;  // the mailbox was delivered
       const double t_relabel = std::chrono::duration<double>(clk::now() - tb).count();
       stats.bfs_seconds += t_relabel;
       ++stats.cycles;
       ++cycles;
-      if (trace_)
-        std::fprintf(stderr, "[lf] cycle %lld active %lld bfs_launches %lld tiles_pushed %llu R %d\n", cycles, active,
-                     stats.bfs_launches - bl0, (unsigned long long)read_tiles(), R);
-      if (active == 0) break;
+      if (trace_) {
+        hipLaunchKernelGGL((k_max_label), dim3(red_blocks()), dim3(NTHREADS), 0, s_, h_, N_, part_);
+        hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
+        const unsigned long long mxl = read_counter();
+        std::fprintf(stderr, "[lf] cycle %lld active %lld bfs_launches %lld max_label %llu tiles_pushed %llu R %d\n",
+                     cycles, active, stats.bfs_launches - bl0, mxl, (unsigned long long)read_tiles(), R);
+      }
+      if (active == 0) {
+        if (exact) break;
+        need_exact = true;  // confirm with an uncapped relabel before stopping
+        continue;
+      }
+      need_exact = false;
       if (tun.max_cycles > 0 && cycles > tun.max_cycles)
         fail(LF_ERR_STATE, "max_cycles reached before the flow became maximum");
       p.iters = tun.local_iters > 0 ? tun.local_iters : (active * 64 < N_ ? 16 : 4);
@@ -1284,7 +1358,10 @@ class Engine final : public MinCutEngine {
           KP<T> q = p;
           q.flag_in = flags_[cur_];
           q.flag_out = flags_[cur_ ^ 1];
-          hipLaunchKernelGGL((k_push<K, G, T>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q);
+          with_geo([&](auto g) {
+            using GG = decltype(g);
+            hipLaunchKernelGGL((k_push<K, GG, T>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q);
+          });
           cur_ ^= 1;
           ++stats.rounds;
         }
@@ -1319,7 +1396,7 @@ class Engine final : public MinCutEngine {
     hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, tiles_ptr(), 0ull, 1LL);
   }
   void end_stats() {
-    stats.tile_nodes = G::NT;
+    stats.tile_nodes = grid_[variant_].nt;
     stats.bytes_per_node = 2 * ((long long)K * sizeof(T) + sizeof(T) + 4);
     LF_HIP(hipMemcpy(host_small_, tiles_ptr(), sizeof(unsigned long long), hipMemcpyDeviceToHost));
     stats.tiles_pushed = (long long)*reinterpret_cast<unsigned long long*>(host_small_);
@@ -1514,10 +1591,12 @@ class Engine final : public MinCutEngine {
   // batches; only the last launch of a batch is checked, launches over
   // converged tiles exit at their flag test.
   template <class F>
-  void run_tiled(F&& launch) {
+  bool run_tiled(F&& launch, long long cap = 0) {
     fill_flags(bflags_[bcur_], 1, bntiles_);
     fill_flags(bflags_[bcur_ ^ 1], 0, bntiles_);
+    long long launched = 0;
     for (int batch = 2;; batch = std::min(batch * 2, 8)) {
+      if (cap > 0 && launched >= cap) return false;  // stopped before the fixpoint
       for (int b = 0; b < batch; ++b) {
         if (b == batch - 1) zero_counter();
         BP q = bp_;
@@ -1526,27 +1605,38 @@ class Engine final : public MinCutEngine {
         q.counter = counter_ptr();
         launch(q);
         bcur_ ^= 1;
+        ++launched;
       }
       LF_HIP(hipGetLastError());
-      if (read_counter() == 0) break;
+      if (read_counter() == 0) return true;
     }
   }
 
-  // deliver mailboxes, exact BFS labels, flag active tiles; returns #active nodes
-  long long global_relabel(KP<T> p, bool pushed) {
+  // Deliver mailboxes, BFS labels, flag the tiles of active nodes; returns the
+  // number of active nodes.  With cap > 0 the BFS may stop after cap launches:
+  // the labels are then lengths of real residual paths (upper bounds of the
+  // distances) and unreached nodes are INF, which only steers pushes; *exact
+  // tells whether the fixpoint was reached.
+  long long global_relabel(KP<T> p, bool pushed, long long cap, bool* exact) {
     if (pushed)
-      hipLaunchKernelGGL((k_deliver<K, G, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
+      with_geo([&](auto g) {
+        using GG = decltype(g);
+        hipLaunchKernelGGL((k_deliver<K, GG, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
+      });
     hipLaunchKernelGGL((k_bfs_init<K, T, MT>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, mask_);
-    run_tiled([&](const BP& q) {
+    *exact = run_tiled([&](const BP& q) {
       hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, h_);
       ++stats.bfs_launches;
-    });
+    }, cap);
     // push rounds start with exactly the tiles holding active nodes
     fill_flags(flags_[cur_], 0);
     fill_flags(flags_[cur_ ^ 1], 0);
     KP<T> q = p;
     q.flag_out = flags_[cur_];
-    hipLaunchKernelGGL((k_count_active<G, T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, q, part_);
+    with_geo([&](auto g) {
+      using GG = decltype(g);
+      hipLaunchKernelGGL((k_count_active<GG, T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, q, part_);
+    });
     hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
     LF_HIP(hipGetLastError());
     return (long long)read_counter();
@@ -1567,6 +1657,8 @@ class Engine final : public MinCutEngine {
   int* flags_[2] = {nullptr, nullptr};
   int cur_ = 0;
   BP bp_{};
+  TileGrid grid_[3];
+  int variant_ = 0;
   long long bntiles_ = 0;
   int* bflags_[2] = {nullptr, nullptr};
   int bcur_ = 0;
@@ -1582,12 +1674,12 @@ class Engine final : public MinCutEngine {
   int* pf_nreg_ = nullptr;
 };
 
-template <int K, class G, class B>
+template <int K, class G0, class G1, class G2, class B>
 std::unique_ptr<MinCutEngine> make_k(const LatticeGeom& g, int cap_bytes) {
   using MT = typename std::conditional<(K <= 8), unsigned char,
              typename std::conditional<(K <= 16), unsigned short, unsigned int>::type>::type;
-  if (cap_bytes == 4) return std::unique_ptr<MinCutEngine>(new Engine<K, G, B, int, MT>(g));
-  return std::unique_ptr<MinCutEngine>(new Engine<K, G, B, long long, MT>(g));
+  if (cap_bytes == 4) return std::unique_ptr<MinCutEngine>(new Engine<K, G0, G1, G2, B, int, MT>(g));
+  return std::unique_ptr<MinCutEngine>(new Engine<K, G0, G1, G2, B, long long, MT>(g));
 }
 
 int max_abs(const std::vector<int>& v) {
@@ -1603,23 +1695,27 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
   const int rz = max_abs(g.dz), ry = max_abs(g.dy), rx = max_abs(g.dx);
   const bool is3d = g.ndim == 3;
   if (!is3d && rz == 0 && ry <= 1 && rx <= 1) {
-    using G = Geo<1, 16, 64, 0, 1, 1>;
+    using G0 = Geo<1, 16, 64, 0, 1, 1>;
+    using G1 = Geo<1, 8, 64, 0, 1, 1>;
+    using G2 = Geo<1, 32, 32, 0, 1, 1>;
     using B = BGeo<1, 64, 64, 0, 1, 1>;
-    if (g.k == 2) return make_k<2, G, B>(g, cap_bytes);
-    if (g.k == 4) return make_k<4, G, B>(g, cap_bytes);
-    if (g.k == 8) return make_k<8, G, B>(g, cap_bytes);
+    if (g.k == 2) return make_k<2, G0, G1, G2, B>(g, cap_bytes);
+    if (g.k == 4) return make_k<4, G0, G1, G2, B>(g, cap_bytes);
+    if (g.k == 8) return make_k<8, G0, G1, G2, B>(g, cap_bytes);
   }
   if (!is3d && rz == 0 && ry <= 2 && rx <= 2) {
-    using G = Geo<1, 16, 64, 0, 2, 2>;
+    using G0 = Geo<1, 16, 64, 0, 2, 2>;
+    using G1 = Geo<1, 8, 64, 0, 2, 2>;
+    using G2 = Geo<1, 32, 32, 0, 2, 2>;
     using B = BGeo<1, 64, 64, 0, 2, 2>;
-    if (g.k == 12) return make_k<12, G, B>(g, cap_bytes);
-    if (g.k == 16) return make_k<16, G, B>(g, cap_bytes);
+    if (g.k == 12) return make_k<12, G0, G1, G2, B>(g, cap_bytes);
+    if (g.k == 16) return make_k<16, G0, G1, G2, B>(g, cap_bytes);
   }
   if (is3d && rz <= 1 && ry <= 1 && rx <= 1) {
     using B = BGeo<16, 16, 16, 1, 1, 1>;
-    if (g.k == 6) return make_k<6, Geo<4, 8, 32, 1, 1, 1>, B>(g, cap_bytes);
-    if (g.k == 18) return make_k<18, Geo<4, 4, 16, 1, 1, 1>, B>(g, cap_bytes);
-    if (g.k == 26) return make_k<26, Geo<4, 4, 16, 1, 1, 1>, B>(g, cap_bytes);
+    if (g.k == 6) return make_k<6, Geo<4, 8, 32, 1, 1, 1>, Geo<2, 8, 32, 1, 1, 1>, Geo<4, 16, 16, 1, 1, 1>, B>(g, cap_bytes);
+    if (g.k == 18) return make_k<18, Geo<4, 4, 16, 1, 1, 1>, Geo<2, 8, 16, 1, 1, 1>, Geo<4, 8, 8, 1, 1, 1>, B>(g, cap_bytes);
+    if (g.k == 26) return make_k<26, Geo<4, 4, 16, 1, 1, 1>, Geo<2, 8, 16, 1, 1, 1>, Geo<4, 8, 8, 1, 1, 1>, B>(g, cap_bytes);
   }
   fail(LF_ERR_UNSUPPORTED, "no device path for this kernel size / radius yet");
 }

@@ -191,9 +191,20 @@ class LatticeFlow(object):
     def resetEnergy(self):
         check(lib().lf_reset_energies(self._h))
 
-    def setTuning(self, rounds_per_relabel=0, local_iters=0, max_cycles=0):
-        t = _lib.lf_tuning(int(rounds_per_relabel), int(local_iters), int(max_cycles), 0)
+    def setTuning(self, rounds_per_relabel=0, local_iters=0, max_cycles=0, tile_shape=0):
+        t = _lib.lf_tuning(int(rounds_per_relabel), int(local_iters), int(max_cycles), int(tile_shape))
         check(lib().lf_set_tuning(self._h, ctypes.byref(t)))
+
+    def getTuning(self):
+        t = _lib.lf_tuning()
+        check(lib().lf_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, reps=1):
+        """Time every tile shape / relabel schedule on the current energies and keep the fastest."""
+        t = _lib.lf_tuning()
+        check(lib().lf_autotune(self._h, int(reps), ctypes.byref(t)))
+        return t.as_dict()
 
     def run(self):
         """Runs the specified solver (pyx:269-274)."""

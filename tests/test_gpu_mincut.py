@@ -78,3 +78,24 @@ def test_blobs26_small(oracle):
     from latticeflow_amd import workloads as W
     w = W.blobs26(seed=7, shape=(40, 48, 56))
     _check(oracle, w["kernel"], w["shape"], w["unary"], w["pairwise"], cap_bytes=8)
+
+
+@pytest.mark.parametrize("tile_shape", [1, 2, 3])
+@pytest.mark.parametrize("kernel,shape", [("Full2d_8", (150, 210)), ("Star2d_16", (77, 130)), ("Star3d_26", (20, 33, 41))])
+def test_tile_shapes(oracle, tile_shape, kernel, shape):
+    rng = np.random.default_rng(tile_shape * 31 + len(shape))
+    offs, _ = oracle.edge_order(kernel, shape)
+    un, b = _random_energies(rng, shape, offs, cap=200, wcap=60)
+    for tun in ({"tile_shape": tile_shape}, {"tile_shape": tile_shape, "rounds_per_relabel": 3, "local_iters": 1}):
+        _check(oracle, kernel, shape, un, b, cap_bytes=4, tuning=tun)
+
+
+def test_autotune_keeps_results(oracle):
+    from latticeflow_amd import workloads as W
+    w = W.seg8(seed=9, shape=(300, 400))
+    flow, smin, smax, lf = _solve_gpu(w["kernel"], w["shape"], w["unary"], w["pairwise"], cap_bytes=4)
+    best = lf.autotune(reps=1)
+    assert 1 <= best["tile_shape"] <= 3
+    lf.run()
+    assert lf.flow() == flow
+    assert np.array_equal(lf.getCut().reshape(-1), smin)
