@@ -52,6 +52,8 @@ def main():
     a = ap.parse_args()
     if a.workload == "tvpath":
         return bench_tvpath(a)
+    if a.workload == "batch512":
+        return bench_batch(a)
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the solver has no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
@@ -160,6 +162,101 @@ def main():
                    "flow": flow, "parity_vs_cpu": parity,
                    "cycles": st["cycles"], "rounds": st["rounds"], "bfs_launches": st["bfs_launches"],
                    "tuning": tuning},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(a):
+    """Batched min-cuts: 512 Full2d_8 segmentation lattices of uneven sizes
+    (sides log-uniform in [64, 1024], aspect ratio in [0.5, 2], each lattice
+    with its own seed), packed on one device canvas and solved by one min-cut.
+    A step is one lf_run over the whole batch; value = sum of edges / step time."""
+    import numpy as np
+    import torch  # noqa: F401
+    from concurrent.futures import ThreadPoolExecutor
+    from latticeflow_amd import LatticeFlowBatch, workloads as W
+    count = int(a.shape) if a.shape else 512
+    inst = W.batch_instances(seed=4, count=count)
+    shapes = [s for s, _ in inst]
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    bt = LatticeFlowBatch("Full2d_8", shapes, numerical_precision=1.0, cap_bytes=4)
+    offs = None
+    edges = 0
+    keep = {}
+    sample = list(range(0, count, max(1, count // 128)))[:128]
+    with ThreadPoolExecutor(threads) as ex:
+        for i, w in enumerate(ex.map(lambda t: W.seg8(seed=t[1], shape=t[0]), inst)):
+            bt.addEnergyInt(i, w["unary"], w["pairwise"])
+            if offs is None:
+                offs = bt.edges(i)
+            e = bt.edges(i)
+            edges += W.n_edges(np.concatenate([e, -e]), shapes[i])
+            if i in sample and not a.no_cpu_baseline:
+                keep[i] = w
+    if a.verbose:
+        print("generated and ingested %d lattices in %.1fs, canvas %s" % (count, time.time() - t0, bt.canvas()),
+              file=sys.stderr, flush=True)
+    if a.rounds or a.iters or a.tile_shape:
+        bt.setTuning(rounds_per_relabel=a.rounds, local_iters=a.iters, tile_shape=a.tile_shape)
+    elif not a.no_autotune:
+        bt.autotune(reps=2)
+    tuning = bt.getTuning()
+    for _ in range(a.warmup):
+        bt.run()
+    push_s = 0.0
+    tiles = 0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        bt.run()
+        flows = bt.flows()
+        st = bt.stats()
+        push_s += st["push_seconds"]
+        tiles += st["tiles_pushed"]
+        launches += st["rounds"]
+    dt = (time.perf_counter() - t0) / a.steps
+    bytes_alg = tiles * st["tile_nodes"] * st["bytes_per_node"]
+    achieved = bytes_alg / push_s / 1e9 if push_s > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "k_push (push/relabel round)", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_us": round(push_s / max(launches, 1) * 1e6, 2), "bytes_per_node": st["bytes_per_node"],
+                "nodes_per_launch": round(tiles * st["tile_nodes"] / max(launches, 1), 1),
+                "push_share_of_step": round(push_s / a.steps / dt, 3)}
+    cpu = None
+    parity = None
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        prep = {}
+        for i, w in keep.items():
+            o = np.concatenate([bt.edges(i), -bt.edges(i)])
+            prep[i] = (w["shape"], o) + O.reparam(w["shape"], o, w["unary"], w["pairwise"])
+        tc = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            refs = dict(zip(prep, ex.map(lambda i: O.prflow(*prep[i]), prep)))
+        tcpu = time.perf_counter() - tc
+        sedges = sum(W.n_edges(prep[i][1], prep[i][0]) for i in prep)
+        cuts = bt.getCuts()
+        parity = all(refs[i]["flow"] == flows[i] and np.array_equal(cuts[i].reshape(-1), refs[i]["smin"].astype(np.int32))
+                     for i in refs)
+        cpu = {"value": round(sedges / tcpu / 1e6, 3), "unit": "Medges/s", "cores": threads, "kind": "port",
+               "sample": "%d of the %d lattices (every %dth, %d edges), one PRFlow-restatement solve each "
+                         "(oracle/lf_oracle.c) on a pool of %d threads, %.2f s"
+                         % (len(prep), count, max(1, count // 128), sedges, threads, tcpu)}
+    hc, wc = bt.canvas()
+    out = {
+        "metric": "Batched min-cut Medges/s (%d Full2d_8 lattices, sides 64..1024)" % count,
+        "value": round(edges / dt / 1e6, 3), "unit": "Medges/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "batch%d seg8 Full2d_8 (seeds 40000+i)" % count, "lattices": count,
+                   "edges": edges, "nodes": bt.size, "canvas": [hc, wc],
+                   "packing": round(bt.size / (hc * wc), 4), "flow_sum": int(flows.sum()),
+                   "parity_vs_cpu_sample": parity, "cycles": st["cycles"], "rounds": st["rounds"],
+                   "bfs_launches": st["bfs_launches"], "tuning": tuning},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -59,6 +59,16 @@ typedef struct {
   double flood_seconds;  /* wall time of the S_min flood */
 } lf_stats;
 
+/* Statistics of a guided divide and conquer (lf_tv_path_guided). */
+typedef struct lf_guided_stats {
+  int64_t regions;          /* regions of the initial partition */
+  int64_t repairs;          /* checks that found failing cut arcs (each followed by a merge) */
+  int64_t failing_arcs;     /* failing cut arcs over all checks */
+  int64_t layers_merged;    /* checks after which every still-failing layer was merged whole */
+  int64_t fallback;         /* 1 when the partition was dropped (region table full) */
+  double partition_seconds; /* building the partition and its network */
+} lf_guided_stats;
+
 /* Message of the last failure on this thread ("" when none). */
 const char *lf_last_error(void);
 
@@ -137,6 +147,32 @@ int lf_get_cut_device(const lf_handle *h, int32_t *d_out, int which, void *strea
  * pf_flow_node.hpp:233-235).  Nodes of one level set share r up to +-1. */
 int lf_get_levels(const lf_handle *h, int64_t *r_out);
 
+/* ---- batched lattices ------------------------------------------------------ */
+/* Many independent 2-D lattices of one kernel solved by one device min-cut.
+ * No single call of the reference does this: a reference user creates one
+ * LatticeFlowDirectInterface per lattice (getLatticeFlowInterface,
+ * interface.hpp:171-183) and calls addEnergyPotentials / run / getCut on each
+ * (direct_interface.hpp:49-106); a batch handle gives the same per-lattice
+ * results.  shapes[2*i], shapes[2*i+1] = (n0, n1) of lattice i.  The handle
+ * also takes lf_set_tuning, lf_autotune, lf_run, lf_run_time, lf_get_stats,
+ * lf_reset_energies and lf_destroy; lf_n_nodes is the total node count,
+ * lf_get_cut / lf_get_cut_device write every lattice's cut concatenated in
+ * lattice order (each in C order), lf_get_flow is the sum of the flows. */
+int lf_batch_create(lf_handle **out, const char *kernel, int n_lattices, const int64_t *shapes,
+                    int cap_bytes);
+/* Number of lattices of a batch handle. */
+int lf_batch_size(const lf_handle *h);
+/* Rows and columns of the device canvas the lattices are packed into. */
+int lf_batch_canvas(const lf_handle *h, int64_t *rows_cols);
+/* edgeDeltas of lattice i (its own shape fixes the order). */
+int lf_batch_edge_deltas(const lf_handle *h, int i, int32_t *edge_deltas);
+/* addEnergyPotentials of lattice i, in lattice i's own layout. */
+int lf_batch_add_energy_potentials(lf_handle *h, int i, const int64_t *unary, const int64_t *binary);
+int lf_batch_add_energy_potentials_device(lf_handle *h, int i, const int64_t *d_unary,
+                                          const int64_t *d_binary, void *stream);
+/* Raw max-flow value of every lattice of the last run: flows[n_lattices]. */
+int lf_batch_get_flows(const lf_handle *h, int64_t *flows);
+
 /* ---- total variation: calculate2dTV (solver/tv/tv_solver.hpp:648-681) -- */
 /* Single lambda: calculate2dTV<Kernel, int64>(nx, ny, f, lambda)
  * (tv_solver.hpp:650-655 -> TVSolver::runSingleLambda :151-176).  f and R are
@@ -153,6 +189,20 @@ int lf_tv_single(const char *kernel, int64_t nx, int64_t ny, const double *f, do
  * together on the device as layers of one stacked lattice. */
 int lf_tv_path(const char *kernel, int64_t nx, int64_t ny, const double *f, int64_t n_lambda,
                const double *lambdas, double *R, lf_stats *stats);
+
+/* The same path, solved from an initial partition of every lambda's image:
+ * d_pred (device memory, n_lambda * nx * ny floats, C order per slice) holds a
+ * prediction of the TV output at each lambda; neighbouring pixels whose
+ * predictions differ by at most delta start in one region.  No reference
+ * counterpart: the reference always starts its divide and conquer from the
+ * whole lattice (ParametricFlowSolver::run, pf_solver.hpp:33-208).  Every arc
+ * between two regions is taken as saturated in the order of the predictions
+ * and checked once the regions are solved; regions joined by a failing arc
+ * are merged and solved again (after max_repairs checks, whole lambdas), so
+ * R is the same solution as lf_tv_path's for any prediction. */
+int lf_tv_path_guided(const char *kernel, int64_t nx, int64_t ny, const double *f, int64_t n_lambda,
+                      const double *lambdas, const float *d_pred, double delta, int max_repairs, double *R,
+                      lf_stats *stats, lf_guided_stats *gstats);
 
 #ifdef __cplusplus
 }

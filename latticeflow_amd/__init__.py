@@ -4,7 +4,7 @@ The product is the HIP library ``liblatticeflow_amd.so`` behind the C ABI in
 ``include/latticeflow_amd.h``; this package holds its Python host mirror
 (``pylatticeflow``), the synthetic workload generators and the build script.
 """
-from .pylatticeflow import (LatticeFlow, calculate2dTV, calculate2dTVPath, getKernelDimension,  # noqa: F401
+from .pylatticeflow import (LatticeFlow, LatticeFlowBatch, calculate2dTV, calculate2dTVPath, getKernelDimension,  # noqa: F401
                             validKernelNames, validSolverNames)
 
-__all__ = ["LatticeFlow", "calculate2dTV", "calculate2dTVPath", "getKernelDimension", "validKernelNames", "validSolverNames"]
+__all__ = ["LatticeFlow", "LatticeFlowBatch", "calculate2dTV", "calculate2dTVPath", "getKernelDimension", "validKernelNames", "validSolverNames"]

@@ -56,6 +56,15 @@ class lf_stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class lf_guided_stats(ctypes.Structure):
+    _fields_ = [("regions", ctypes.c_int64), ("repairs", ctypes.c_int64), ("failing_arcs", ctypes.c_int64),
+                ("layers_merged", ctypes.c_int64), ("fallback", ctypes.c_int64),
+                ("partition_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # every symbol the header declares, with its ctypes signature
 _I = ctypes.c_int
 _I64 = ctypes.c_int64
@@ -90,8 +99,17 @@ _SIG = {
     "lf_get_flow": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "lf_get_cut_device": (_I, [_P, _P, _I, _P]),
     "lf_get_levels": (_I, [_P, _P]),
+    "lf_batch_create": (_I, [ctypes.POINTER(_P), _CP, _I, _P, _I]),
+    "lf_batch_size": (_I, [_P]),
+    "lf_batch_canvas": (_I, [_P, _P]),
+    "lf_batch_edge_deltas": (_I, [_P, _I, _P]),
+    "lf_batch_add_energy_potentials": (_I, [_P, _I, _P, _P]),
+    "lf_batch_add_energy_potentials_device": (_I, [_P, _I, _P, _P, _P]),
+    "lf_batch_get_flows": (_I, [_P, _P]),
     "lf_tv_single": (_I, [_CP, _I64, _I64, _P, ctypes.c_double, _P, ctypes.POINTER(lf_stats)]),
     "lf_tv_path": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.POINTER(lf_stats)]),
+    "lf_tv_path_guided": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.c_double, _I, _P,
+                               ctypes.POINTER(lf_stats), ctypes.POINTER(lf_guided_stats)]),
 }
 
 _lib = None

@@ -9,7 +9,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -80,6 +82,10 @@ struct lf_handle {
   LatticeGeom geom;
   std::unique_ptr<MinCutEngine> eng;
   bool levels = false;  // LatticeLevelReductions (parametric divide and conquer)
+  bool batch = false;   // lattices packed on one canvas
+  std::vector<BatchRect> rects;
+  std::vector<std::vector<int>> batch_offs;  // per lattice: k/2 * 2 edge deltas
+  int64_t canvas[2] = {0, 0};
   lf_tuning tuning{};
   double run_time = 0;
 };
@@ -192,6 +198,7 @@ int lf_n_filling_edges(const lf_handle* h) { return h ? h->k / 2 : LF_ERR_BAD_AR
 int lf_edge_deltas(const lf_handle* h, int32_t* ed) {
   return guarded([&] {
     if (!h || !ed) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->batch) fail(LF_ERR_STATE, "a batch handle has one edge order per lattice (lf_batch_edge_deltas)");
     for (int i = 0; i < h->k / 2 * h->ndim; ++i) ed[i] = h->offs[i];
   });
 }
@@ -206,6 +213,7 @@ int lf_directions(const lf_handle* h, int32_t* offs) {
 int lf_add_energy_potentials_device(lf_handle* h, const int64_t* du, const int64_t* db, void* stream) {
   return guarded([&] {
     if (!h || !du || !db) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->batch) fail(LF_ERR_STATE, "a batch handle takes energies per lattice (lf_batch_add_energy_potentials)");
     h->eng->add_energies_device(du, db, reinterpret_cast<hipStream_t>(stream));
   });
 }
@@ -213,6 +221,7 @@ int lf_add_energy_potentials_device(lf_handle* h, const int64_t* du, const int64
 int lf_add_energy_potentials(lf_handle* h, const int64_t* unary, const int64_t* binary) {
   return guarded([&] {
     if (!h || !unary || !binary) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->batch) fail(LF_ERR_STATE, "a batch handle takes energies per lattice (lf_batch_add_energy_potentials)");
     const size_t nu = (size_t)h->n * 2, nb = (size_t)h->n * (h->k / 2) * 4;
     int64_t *du = nullptr, *db = nullptr;
     hipStream_t s = h->eng->stream();
@@ -322,7 +331,8 @@ int lf_get_cut_device(const lf_handle* h, int32_t* d_out, int which, void* strea
     }
     if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
     if (which != LF_CUT_SMIN && which != LF_CUT_SMAX) fail(LF_ERR_BAD_ARG, "bad cut selector");
-    h->eng->cut_device(d_out, which, reinterpret_cast<hipStream_t>(stream));
+    if (h->batch) h->eng->batch_cut_device(d_out, which, reinterpret_cast<hipStream_t>(stream));
+    else h->eng->cut_device(d_out, which, reinterpret_cast<hipStream_t>(stream));
   });
 }
 
@@ -346,7 +356,8 @@ int lf_get_cut(const lf_handle* h, int32_t* out, int which) {
     } else {
       if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
       if (which != LF_CUT_SMIN && which != LF_CUT_SMAX) fail(LF_ERR_BAD_ARG, "bad cut selector");
-      h->eng->cut_device(d, which, s);
+      if (h->batch) h->eng->batch_cut_device(d, which, s);
+      else h->eng->cut_device(d, which, s);
     }
     LF_HIP(hipMemcpyAsync(out, d, sizeof(int32_t) * h->n, hipMemcpyDeviceToHost, s));
     LF_HIP(hipStreamSynchronize(s));
@@ -383,12 +394,198 @@ int lf_get_flow(const lf_handle* h, int64_t* flow) {
   });
 }
 
+// ---- batched lattices --------------------------------------------------------
+
+// Shelf packing: lattices sorted by height (tallest first) fill rows of a
+// canvas whose width is about the square root of the total area (at least the
+// widest lattice), left to right.
+static void pack_batch(lf_handle* h, const int64_t* shapes, int n) {
+  std::vector<int> idx(n);
+  double area = 0;
+  int64_t wmax = 1;
+  for (int i = 0; i < n; ++i) {
+    idx[i] = i;
+    area += (double)shapes[2 * i] * (double)shapes[2 * i + 1];
+    wmax = std::max(wmax, shapes[2 * i + 1]);
+  }
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+    if (shapes[2 * a] != shapes[2 * b]) return shapes[2 * a] > shapes[2 * b];
+    return shapes[2 * a + 1] > shapes[2 * b + 1];
+  });
+  const int64_t Wc = std::max<int64_t>(wmax, ((int64_t)std::ceil(std::sqrt(area)) + 63) / 64 * 64);
+  int64_t y = 0, x = 0, shelf = 0;
+  h->rects.assign(n, BatchRect{});
+  for (int i : idx) {
+    const int64_t hh = shapes[2 * i], ww = shapes[2 * i + 1];
+    if (x + ww > Wc) {
+      y += shelf;
+      x = 0;
+      shelf = 0;
+    }
+    BatchRect& R = h->rects[i];
+    R.y0 = y;
+    R.x0 = x;
+    R.h = hh;
+    R.w = ww;
+    shelf = std::max(shelf, hh);
+    x += ww;
+  }
+  int64_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    h->rects[i].off = off;
+    off += shapes[2 * i] * shapes[2 * i + 1];
+  }
+  h->canvas[0] = y + shelf;
+  h->canvas[1] = Wc;
+}
+
+int lf_batch_create(lf_handle** out, const char* kernel, int n, const int64_t* shapes, int cap_bytes) {
+  if (out) *out = nullptr;
+  return guarded([&] {
+    if (!out || !kernel || !shapes) fail(LF_ERR_BAD_ARG, "null argument");
+    const KernelDef* kd = find_kernel(kernel);
+    if (!kd) fail(LF_ERR_BAD_NAME, std::string("Bad Kernel or Solver Name: ") + kernel);
+    if (kd->ndim != 2) fail(LF_ERR_UNSUPPORTED, "batches take 2-D kernels");
+    if (n < 1) fail(LF_ERR_BAD_ARG, "a batch needs at least one lattice");
+    if (cap_bytes != 4 && cap_bytes != 8) fail(LF_ERR_BAD_ARG, "cap_bytes must be 4 or 8");
+    std::unique_ptr<lf_handle> h(new lf_handle);
+    h->solver = "LatticeEnergyMinimizer";
+    h->kernel = kernel;
+    h->ndim = 2;
+    h->k = (int)kd->edges.size();
+    h->cap_bytes = cap_bytes;
+    h->batch = true;
+    if (h->k / 2 > 16) fail(LF_ERR_UNSUPPORTED, "kernel too large for a batch");
+    h->n = 0;
+    for (int i = 0; i < n; ++i) {
+      if (shapes[2 * i] <= 0 || shapes[2 * i + 1] <= 0) fail(LF_ERR_BAD_ARG, "every dimension must be positive");
+      if (shapes[2 * i] > (1LL << 20) || shapes[2 * i + 1] > (1LL << 20)) fail(LF_ERR_BAD_ARG, "dimension too large");
+      h->n += shapes[2 * i] * shapes[2 * i + 1];
+    }
+    pack_batch(h.get(), shapes, n);
+    h->shape = {h->canvas[0], h->canvas[1]};
+    if (!lattice_edge_order(*kd, h->shape, h->offs, h->geocut))
+      fail(LF_ERR_BAD_ARG, "kernel is not symmetric on this lattice");
+    const int kp = h->k / 2;
+    h->batch_offs.resize(n);
+    for (int i = 0; i < n; ++i) {
+      std::vector<int> o;
+      std::vector<double> w;
+      if (!lattice_edge_order(*kd, {shapes[2 * i], shapes[2 * i + 1]}, o, w))
+        fail(LF_ERR_BAD_ARG, "kernel is not symmetric on this lattice");
+      h->batch_offs[i].assign(o.begin(), o.begin() + 2 * kp);
+      for (int e = 0; e < kp; ++e) {
+        int found = -1;
+        for (int f = 0; f < kp; ++f)
+          if (o[2 * f] == h->offs[2 * e] && o[2 * f + 1] == h->offs[2 * e + 1]) found = f;
+        if (found < 0) fail(LF_ERR_UNSUPPORTED, "lattice with a different set of positive directions");
+        h->rects[i].perm[e] = (int8_t)found;
+      }
+    }
+    LatticeGeom& g = h->geom;
+    g.ndim = 2;
+    g.k = h->k;
+    g.D = 1;
+    g.H = h->canvas[0];
+    g.W = h->canvas[1];
+    for (int i = 0; i < h->k; ++i) {
+      g.dz.push_back(0);
+      g.dy.push_back(h->offs[i * 2]);
+      g.dx.push_back(h->offs[i * 2 + 1]);
+    }
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+      fail(LF_ERR_DEVICE, "no HIP device available");
+    h->eng = make_mincut_engine(g, cap_bytes);
+    h->eng->batch_set_layout(h->rects);
+    *out = h.release();
+  });
+}
+
+int lf_batch_size(const lf_handle* h) {
+  if (!h) return LF_ERR_BAD_ARG;
+  if (!h->batch) {
+    g_last_error = "not a batch handle";
+    return LF_ERR_STATE;
+  }
+  return (int)h->rects.size();
+}
+
+int lf_batch_canvas(const lf_handle* h, int64_t* rc) {
+  return guarded([&] {
+    if (!h || !rc) fail(LF_ERR_BAD_ARG, "null argument");
+    if (!h->batch) fail(LF_ERR_STATE, "not a batch handle");
+    rc[0] = h->canvas[0];
+    rc[1] = h->canvas[1];
+  });
+}
+
+int lf_batch_edge_deltas(const lf_handle* h, int i, int32_t* ed) {
+  return guarded([&] {
+    if (!h || !ed) fail(LF_ERR_BAD_ARG, "null argument");
+    if (!h->batch) fail(LF_ERR_STATE, "not a batch handle");
+    if (i < 0 || i >= (int)h->rects.size()) fail(LF_ERR_OUT_OF_RANGE, "batch lattice index out of range");
+    for (size_t j = 0; j < h->batch_offs[i].size(); ++j) ed[j] = h->batch_offs[i][j];
+  });
+}
+
+int lf_batch_add_energy_potentials_device(lf_handle* h, int i, const int64_t* du, const int64_t* db, void* stream) {
+  return guarded([&] {
+    if (!h || !du || !db) fail(LF_ERR_BAD_ARG, "null argument");
+    if (!h->batch) fail(LF_ERR_STATE, "not a batch handle");
+    h->eng->batch_add_energies_device(i, du, db, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int lf_batch_add_energy_potentials(lf_handle* h, int i, const int64_t* unary, const int64_t* binary) {
+  return guarded([&] {
+    if (!h || !unary || !binary) fail(LF_ERR_BAD_ARG, "null argument");
+    if (!h->batch) fail(LF_ERR_STATE, "not a batch handle");
+    if (i < 0 || i >= (int)h->rects.size()) fail(LF_ERR_OUT_OF_RANGE, "batch lattice index out of range");
+    const int64_t m = h->rects[i].h * h->rects[i].w;
+    const size_t nu = (size_t)m * 2, nb = (size_t)m * (h->k / 2) * 4;
+    int64_t *du = nullptr, *db = nullptr;
+    hipStream_t s = h->eng->stream();
+    LF_HIP(hipMallocAsync((void**)&du, nu * sizeof(int64_t), s));
+    LF_HIP(hipMallocAsync((void**)&db, nb * sizeof(int64_t), s));
+    struct Free {
+      int64_t *a, *b;
+      hipStream_t s;
+      ~Free() {
+        hipFreeAsync(a, s);
+        hipFreeAsync(b, s);
+        hipStreamSynchronize(s);
+      }
+    } fr{du, db, s};
+    LF_HIP(hipMemcpyAsync(du, unary, nu * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    LF_HIP(hipMemcpyAsync(db, binary, nb * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    h->eng->batch_add_energies_device(i, du, db, s);
+  });
+}
+
+int lf_batch_get_flows(const lf_handle* h, int64_t* flows) {
+  return guarded([&] {
+    if (!h || !flows) fail(LF_ERR_BAD_ARG, "null argument");
+    if (!h->batch) fail(LF_ERR_STATE, "not a batch handle");
+    if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
+    h->eng->batch_flows(flows);
+  });
+}
+
 // Exact TV of one image at several lambdas: the lambdas are the layers of one
 // stacked lattice (no arcs between layers), solved by one device divide and
 // conquer.  r_out[l*n + i] receives the final raw r of pixel i at lambda l.
+// Guidance for tv_solve_layers: predictions of the output per solved layer.
+struct TvGuide {
+  const float* d_pred = nullptr;  // layers * nx * ny floats on the device
+  float delta = 0;
+  int max_repairs = 0;
+  lf_guided_stats* stats = nullptr;
+};
+
 static void tv_solve_layers(const KernelDef& kd, int64_t nx, int64_t ny, const TvBase& b,
                             const std::vector<double>& lams, const lf_tuning& tun, std::vector<int64_t>& r_out,
-                            lf_stats* stats) {
+                            lf_stats* stats, const TvGuide* guide = nullptr) {
   const int layers = (int)lams.size();
   std::vector<int> offs;
   std::vector<double> w;
@@ -415,7 +612,14 @@ static void tv_solve_layers(const KernelDef& kd, int64_t nx, int64_t ny, const T
     }
   std::unique_ptr<MinCutEngine> eng = make_mincut_engine(g, 8);
   eng->set_tv_network(b.uf.data(), pwf.data(), layers);
-  eng->pf_run_layers(tun, layers);
+  if (guide && b.S > 0)
+    // predicted output values x map to r units as the function values do:
+    // r = 2^41 ((x - M) / S + 0.5)
+    eng->pf_run_layers_guided(tun, layers, guide->d_pred, guide->delta, guide->max_repairs,
+                              std::ldexp(1.0, kTvFunctionBits + 1) / b.S,
+                              std::ldexp(1.0, kTvFunctionBits + 1) * (0.5 - b.M / b.S), guide->stats);
+  else
+    eng->pf_run_layers(tun, layers);
   r_out.resize((size_t)layers * nx * ny);
   hipStream_t s = eng->stream();
   int64_t* d = nullptr;
@@ -454,47 +658,89 @@ int lf_tv_single(const char* kernel, int64_t nx, int64_t ny, const double* f, do
   });
 }
 
+static void tv_path_impl(const char* kernel, int64_t nx, int64_t ny, const double* f, int64_t n_lambda,
+                         const double* lambdas, double* R, lf_stats* stats, const float* d_pred, double delta,
+                         int max_repairs, lf_guided_stats* gstats) {
+  const KernelDef& kd = tv_kernel(kernel);
+  if (!f || !R || (n_lambda > 0 && !lambdas)) fail(LF_ERR_BAD_ARG, "null argument");
+  if (nx <= 0 || ny <= 0) fail(LF_ERR_BAD_ARG, "empty image");
+  if (gstats) *gstats = lf_guided_stats{};
+  if (n_lambda <= 0) return;  // the reference returns an empty path (tv_solver.hpp:664-667)
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = nx * ny;
+  // negative values count as 0 (tv_solver.hpp:669-674)
+  std::vector<double> lam(lambdas, lambdas + n_lambda);
+  for (double& l : lam) l = std::max(l, 0.0);
+  const double lmax = std::max(*std::max_element(lam.begin(), lam.end()), 1e-8);
+  TvBase b = tv_base(n, f);
+  // each positive lambda is an exact single-lambda problem; lambda = 0 is the
+  // fully merged solution (constant M)
+  std::vector<double> solve;
+  std::vector<int64_t> which(n_lambda, -1);
+  for (int64_t l = 0; l < n_lambda; ++l)
+    if (lam[l] > 0) {
+      which[l] = (int64_t)solve.size();
+      solve.push_back(lam[l]);
+    }
+  std::vector<int64_t> r;
+  if (!solve.empty()) {
+    if (d_pred) {
+      // predictions of the solved lambdas only, in solve order
+      float* dp = nullptr;
+      hipStream_t s = nullptr;
+      const bool all = (int64_t)solve.size() == n_lambda;
+      if (!all) {
+        LF_HIP(hipMalloc((void**)&dp, sizeof(float) * n * solve.size()));
+        for (int64_t l = 0; l < n_lambda; ++l)
+          if (which[l] >= 0)
+            LF_HIP(hipMemcpyAsync(dp + which[l] * n, d_pred + l * n, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        LF_HIP(hipStreamSynchronize(s));
+      }
+      struct Free {
+        float* p;
+        ~Free() { hipFree(p); }
+      } fr{dp};
+      TvGuide gd;
+      gd.d_pred = all ? d_pred : dp;
+      gd.delta = (float)delta;
+      gd.max_repairs = max_repairs;
+      gd.stats = gstats;
+      tv_solve_layers(kd, nx, ny, b, solve, lf_tuning{}, r, stats, &gd);
+    } else {
+      tv_solve_layers(kd, nx, ny, b, solve, lf_tuning{}, r, stats);
+    }
+  }
+  std::vector<double> rp((size_t)n);
+  for (int64_t l = 0; l < n_lambda; ++l) {
+    double* out = R + l * n;
+    if (which[l] < 0) {
+      for (int64_t i = 0; i < n; ++i) out[i] = b.M;
+      continue;
+    }
+    // path value at s = lambda / lambda_max: r_path(s) = s r(lambda) by
+    // homogeneity of the TV problem (TVRegPathSegment::get_r_AtLambda,
+    // tv_regpath_node.hpp:146-150, is linear in s over each segment)
+    const double sc = lam[l] / lmax;
+    const int64_t* rl = r.data() + which[l] * n;
+    for (int64_t i = 0; i < n; ++i) rp[i] = sc * (double)rl[i];
+    tv_output_path(b, rp.data(), n, out);
+  }
+  if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 int lf_tv_path(const char* kernel, int64_t nx, int64_t ny, const double* f, int64_t n_lambda,
                const double* lambdas, double* R, lf_stats* stats) {
+  return guarded([&] { tv_path_impl(kernel, nx, ny, f, n_lambda, lambdas, R, stats, nullptr, 0, 0, nullptr); });
+}
+
+int lf_tv_path_guided(const char* kernel, int64_t nx, int64_t ny, const double* f, int64_t n_lambda,
+                      const double* lambdas, const float* d_pred, double delta, int max_repairs, double* R,
+                      lf_stats* stats, lf_guided_stats* gstats) {
   return guarded([&] {
-    const KernelDef& kd = tv_kernel(kernel);
-    if (!f || !R || (n_lambda > 0 && !lambdas)) fail(LF_ERR_BAD_ARG, "null argument");
-    if (nx <= 0 || ny <= 0) fail(LF_ERR_BAD_ARG, "empty image");
-    if (n_lambda <= 0) return;  // the reference returns an empty path (tv_solver.hpp:664-667)
-    const auto t0 = std::chrono::steady_clock::now();
-    const int64_t n = nx * ny;
-    // negative values count as 0 (tv_solver.hpp:669-674)
-    std::vector<double> lam(lambdas, lambdas + n_lambda);
-    for (double& l : lam) l = std::max(l, 0.0);
-    const double lmax = std::max(*std::max_element(lam.begin(), lam.end()), 1e-8);
-    TvBase b = tv_base(n, f);
-    // each positive lambda is an exact single-lambda problem; lambda = 0 is the
-    // fully merged solution (constant M)
-    std::vector<double> solve;
-    std::vector<int64_t> which(n_lambda, -1);
-    for (int64_t l = 0; l < n_lambda; ++l)
-      if (lam[l] > 0) {
-        which[l] = (int64_t)solve.size();
-        solve.push_back(lam[l]);
-      }
-    std::vector<int64_t> r;
-    if (!solve.empty()) tv_solve_layers(kd, nx, ny, b, solve, lf_tuning{}, r, stats);
-    std::vector<double> rp((size_t)n);
-    for (int64_t l = 0; l < n_lambda; ++l) {
-      double* out = R + l * n;
-      if (which[l] < 0) {
-        for (int64_t i = 0; i < n; ++i) out[i] = b.M;
-        continue;
-      }
-      // path value at s = lambda / lambda_max: r_path(s) = s r(lambda) by
-      // homogeneity of the TV problem (TVRegPathSegment::get_r_AtLambda,
-      // tv_regpath_node.hpp:146-150, is linear in s over each segment)
-      const double sc = lam[l] / lmax;
-      const int64_t* rl = r.data() + which[l] * n;
-      for (int64_t i = 0; i < n; ++i) rp[i] = sc * (double)rl[i];
-      tv_output_path(b, rp.data(), n, out);
-    }
-    if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!d_pred) fail(LF_ERR_BAD_ARG, "null prediction");
+    if (!(delta >= 0)) fail(LF_ERR_BAD_ARG, "delta must be >= 0");
+    if (max_repairs < 0) fail(LF_ERR_BAD_ARG, "max_repairs must be >= 0");
+    tv_path_impl(kernel, nx, ny, f, n_lambda, lambdas, R, stats, d_pred, delta, max_repairs, gstats);
   });
 }
 

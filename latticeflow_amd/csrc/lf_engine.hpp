@@ -38,6 +38,15 @@ struct LatticeGeom {
   int64_t n() const { return D * H * W; }
 };
 
+// One lattice of a batch: a (h, w) 2-D lattice placed at (y0, x0) of the
+// engine's canvas; its nodes are entries [off, off + h*w) of the batch's
+// concatenated node order.  perm[e] = this lattice's edge index of the
+// canvas's positive direction e (the lattice order depends on the shape).
+struct BatchRect {
+  int64_t y0, x0, h, w, off;
+  int8_t perm[16];
+};
+
 class MinCutEngine {
  public:
   virtual ~MinCutEngine() {}
@@ -55,6 +64,15 @@ class MinCutEngine {
   // The same over a stack of `layers` equal lattices along the slowest axis
   // (no arcs between layers): one initial region per layer.
   virtual void pf_run_layers(const lf_tuning& tuning, int layers) = 0;
+  // The same from an initial partition: regions of neighbouring nodes whose
+  // predicted values (d_pred, one float per node) differ by at most delta,
+  // with the cut arcs checked afterwards and failing regions merged (at most
+  // max_repairs times; then whole layers).  The result is the unique solution.
+  // Predicted values also choose thresholds: a region cuts at the emptiest gap
+  // of its predicted values (in r units: pred * pscale + poff) while it has
+  // one, and cuts at its mean (the plain rule) once all gaps are used.
+  virtual void pf_run_layers_guided(const lf_tuning& tuning, int layers, const float* d_pred, float delta,
+                                    int max_repairs, double pscale, double poff, lf_guided_stats* gs) = 0;
   // TV network of a stack of images: uf[H*W] per-pixel function values, pwf[layers][k/2]
   // per-direction pairwise values (tv_solver.hpp:71-112).
   virtual void set_tv_network(const int64_t* uf, const int64_t* pwf, int layers) = 0;
@@ -62,6 +80,13 @@ class MinCutEngine {
   virtual void pf_cut_device(int32_t* d_out, hipStream_t s) = 0;  // r < 0
   // Replace the base network (host arrays: red[N], alpha[N*k] node-major, lattice order).
   virtual void set_base_network(const int64_t* red, const int64_t* alpha) = 0;
+  // Batched lattices on a 2-D canvas (arcs between lattices stay zero).
+  virtual void batch_set_layout(const std::vector<BatchRect>& rects) = 0;
+  // Energies of lattice i in its own layout (device pointers).
+  virtual void batch_add_energies_device(int i, const int64_t* d_unary, const int64_t* d_binary, hipStream_t s) = 0;
+  virtual void batch_flows(int64_t* host_flows) = 0;  // raw flow of every lattice of the last run
+  // Cuts of every lattice, concatenated in batch node order.
+  virtual void batch_cut_device(int32_t* d_out, int which, hipStream_t s) = 0;
   int64_t flow = 0;
   bool solved = false;
   bool pf_solved = false;

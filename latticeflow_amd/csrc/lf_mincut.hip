@@ -780,6 +780,114 @@ __global__ __launch_bounds__(NTHREADS) void k_cut(KP<T> p, const int* __restrict
   out[v] = which == LF_CUT_SMAX ? (p.h[v] >= INF_H ? 1 : 0) : reach[v];
 }
 
+// ---- batched lattices on one canvas ------------------------------------------
+//
+// Lattice i of a batch owns the canvas rectangle rects[i]; a flat list of
+// blocks covers every lattice in chunks of BCHUNK nodes (blk[i] = first block
+// of lattice i), so the per-lattice loops touch no filler node.
+
+constexpr int BCHUNK = NTHREADS * 8;
+
+__device__ inline int batch_block_owner(const long long* __restrict__ blk, int n) {
+  __shared__ int s_owner;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = n - 1;  // largest i with blk[i] <= blockIdx.x
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (blk[mid] <= (long long)blockIdx.x) lo = mid;
+      else hi = mid - 1;
+    }
+    s_owner = lo;
+  }
+  __syncthreads();
+  return s_owner;
+}
+
+// flow of every lattice: sum(max(red0,0)) - sum(max(-ex,0)) over its nodes
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_batch_flow(KP<T> p, const long long* __restrict__ red0,
+                                                         const BatchRect* __restrict__ rects,
+                                                         const long long* __restrict__ blk, int n,
+                                                         unsigned long long* __restrict__ flows) {
+  const int i = batch_block_owner(blk, n);
+  const BatchRect R = rects[i];
+  const long long m = R.h * R.w;
+  const long long j0 = ((long long)blockIdx.x - blk[i]) * BCHUNK;
+  const long long j1 = j0 + BCHUNK < m ? j0 + BCHUNK : m;
+  long long f = 0;
+  for (long long j = j0 + threadIdx.x; j < j1; j += NTHREADS) {
+    const long long v = (R.y0 + j / R.w) * p.W + R.x0 + j % R.w;
+    const long long r = red0[v];
+    const long long e = (long long)p.ex[v];
+    f += (r > 0 ? r : 0) - (e < 0 ? -e : 0);
+  }
+  f = block_reduce<false>(f);
+  if (threadIdx.x == 0) atomicAdd(&flows[i], (unsigned long long)f);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_batch_cut(KP<T> p, const int* __restrict__ reach, int which,
+                                                        const BatchRect* __restrict__ rects,
+                                                        const long long* __restrict__ blk, int n,
+                                                        int* __restrict__ out) {
+  const int i = batch_block_owner(blk, n);
+  const BatchRect R = rects[i];
+  const long long m = R.h * R.w;
+  const long long j0 = ((long long)blockIdx.x - blk[i]) * BCHUNK;
+  const long long j1 = j0 + BCHUNK < m ? j0 + BCHUNK : m;
+  for (long long j = j0 + threadIdx.x; j < j1; j += NTHREADS) {
+    const long long v = (R.y0 + j / R.w) * p.W + R.x0 + j % R.w;
+    out[R.off + j] = which == LF_CUT_SMAX ? (p.h[v] >= INF_H ? 1 : 0) : reach[v];
+  }
+}
+
+// Energies of one lattice of the batch (its own layout, rows of R.w nodes)
+// into the canvas network; the gather form of k_reparam over local
+// coordinates, so arcs that would leave the lattice are never written.
+template <int K>
+__global__ __launch_bounds__(NTHREADS) void k_batch_reparam(BatchRect R, int Wc, long long N, DirTab dirs,
+                                                            const long long* __restrict__ unary,
+                                                            const long long* __restrict__ binary,
+                                                            long long* __restrict__ red0,
+                                                            long long* __restrict__ alpha0, int* err) {
+  const long long j = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (j >= R.h * R.w) return;
+  constexpr int KP2 = K / 2;
+  const int ly = (int)(j / R.w), lx = (int)(j % R.w);
+  const long long v = (R.y0 + ly) * (long long)Wc + R.x0 + lx;
+  long long r = 2 * (unary[2 * j + 1] - unary[2 * j + 0]);
+  bool bad = false;
+#pragma unroll
+  for (int e = 0; e < KP2; ++e) {
+    const int dy = dirs.dy[e], dx = dirs.dx[e], pe = R.perm[e];
+    {
+      const int ny = ly + dy, nx = lx + dx;
+      if (ny >= 0 && ny < R.h && nx >= 0 && nx < R.w) {
+        const long long* E = binary + (j * KP2 + pe) * 4;
+        const long long e00 = E[0], e01 = E[1], e10 = E[2], e11 = E[3];
+        const long long q1 = e10 - e00, q12 = e00 + e11 - e01 - e10;
+        bad |= q12 > 0;
+        r += 2 * q1 + q12;
+        alpha0[(long long)e * N + v] += -q12;
+      }
+    }
+    {
+      const int ny = ly - dy, nx = lx - dx;
+      if (ny >= 0 && ny < R.h && nx >= 0 && nx < R.w) {
+        const long long w = (long long)ny * R.w + nx;
+        const long long* E = binary + (w * KP2 + pe) * 4;
+        const long long e00 = E[0], e01 = E[1], e10 = E[2], e11 = E[3];
+        const long long q2 = e01 - e00, q12 = e00 + e11 - e01 - e10;
+        bad |= q12 > 0;
+        r += 2 * q2 + q12;
+        alpha0[(long long)(e + KP2) * N + v] += -q12;
+      }
+    }
+  }
+  red0[v] += r;
+  if (bad) atomicOr(err, 1);
+}
+
 // ---- energies -> network (NodeFiller::addE1/addE2, network_flow_node.hpp:419-485) ----
 
 // Gather form: every node reads the pairwise terms of the edges it belongs to,
@@ -881,8 +989,15 @@ struct RegionStat {
   unsigned long long mn, mx;          // order-preserving encodings of min / max r
   unsigned long long cnt, cnt1;       // nodes; nodes on the source side after the cut
   long long thr;                      // threshold of this level
-  int state;                          // 1 active, 2 final
+  int state;                          // 1 active, 2 final, 3 waiting (no cut at this depth)
   int newid;                          // id of the split-off source-side part, -1 when none
+  unsigned long long pmn, pmx;        // order-preserving min / max of the predicted values
+  int mode;                           // 0: predicted thresholds allowed, 1: mean thresholds only
+  int hslot;                          // histogram of this depth, -1 when none
+  int kind;                           // this depth's cut: 0 at the mean, 1 at a predicted gap
+  int fresh;                          // threshold set since the last setup of excesses
+  int busy;                           // active nodes seen at the last relabel
+  int pad_;
 };
 
 struct PFP {
@@ -890,9 +1005,22 @@ struct PFP {
   int* reg;
   RegionStat* tab;
   int* nreg;
+  int* err;       // bit 0: the region table is full
   long long N;
+  long long cap;  // entries of tab
   int D, H, W;
+  int collapse;   // a region that a cut at its mean does not split is final
+  // thresholds from predictions: predicted value of node v in r units is
+  // pred[v] * pscale + poff; hist holds nbins counters per histogram slot
+  const float* pred;
+  double pscale, poff;
+  unsigned* hist;
+  int nbins;
 };
+
+__device__ inline long long pred_r(const PFP& f, long long v) {
+  return (long long)floor((double)f.pred[v] * f.pscale + f.poff);
+}
 
 __device__ inline unsigned long long ord64(long long x) { return (unsigned long long)x ^ 0x8000000000000000ull; }
 __device__ inline long long unord64(unsigned long long u) { return (long long)(u ^ 0x8000000000000000ull); }
@@ -933,11 +1061,12 @@ __device__ inline double i128_to_double(unsigned long long hi, unsigned long lon
   return neg ? -d : d;
 }
 
-__global__ __launch_bounds__(NTHREADS) void k_pf_clear(PFP f, int nreg) {
+__global__ __launch_bounds__(NTHREADS) void k_pf_clear(PFP f, int nreg, int st) {
   const int k = blockIdx.x * NTHREADS + threadIdx.x;
   if (k >= nreg) return;
   RegionStat& t = f.tab[k];
-  if (t.state != 1) return;
+  if (t.state == 3) t.state = 1;
+  if (t.state != st) return;
   t.sum_lo = 0;
   t.sum_hi = 0;
   t.mn = ~0ull;
@@ -945,14 +1074,19 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_clear(PFP f, int nreg) {
   t.cnt = 0;
   t.cnt1 = 0;
   t.newid = -1;
+  t.pmn = ~0ull;
+  t.pmx = 0;
+  t.hslot = -1;
+  t.kind = 0;
 }
 
 // Per-region aggregation of the active nodes.  MODE 0: sum / min / max / count
-// of r.  MODE 1: count of the nodes with side[v] != 0.  Each block folds its
+// of r.  MODE 1: count of the nodes with side[v] != 0.  MODE 2: min / max of
+// the predicted values.  Each block folds its
 // nodes into an LDS hash table keyed by region (spatially coherent, so a block
 // sees few regions) and flushes it with one set of atomics per region.
 template <int MODE>
-__global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __restrict__ side) {
+__global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __restrict__ side, int st) {
   constexpr int HSZ = 256;
   __shared__ int hk[HSZ];
   __shared__ long long hs[HSZ];
@@ -969,8 +1103,8 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __
   for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < f.N; v += (long long)gridDim.x * NTHREADS) {
     const int k = f.reg[v];
     RegionStat* t = &f.tab[k];
-    if (t->state != 1) continue;
-    const long long x = MODE == 0 ? f.r[v] : (long long)(side[v] != 0);
+    if (t->state != st) continue;
+    const long long x = MODE == 0 ? f.r[v] : MODE == 2 ? pred_r(f, v) : (long long)(side[v] != 0);
     if (MODE == 1 && x == 0) continue;
     int h = (int)(((unsigned)k * 2654435761u) >> 24) & (HSZ - 1);
     int slot = -1;
@@ -984,8 +1118,8 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __
       h = (h + 1) & (HSZ - 1);
     }
     if (slot >= 0) {
-      if (MODE == 0) {
-        atomicAdd((unsigned long long*)&hs[slot], (unsigned long long)x);
+      if (MODE == 0) atomicAdd((unsigned long long*)&hs[slot], (unsigned long long)x);
+      if (MODE != 1) {
         atomicMin(&hmn[slot], ord64(x));
         atomicMax(&hmx[slot], ord64(x));
       }
@@ -996,6 +1130,9 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __
         atomicMin(&t->mn, ord64(x));
         atomicMax(&t->mx, ord64(x));
         atomicAdd(&t->cnt, 1ull);
+      } else if (MODE == 2) {
+        atomicMin(&t->pmn, ord64(x));
+        atomicMax(&t->pmx, ord64(x));
       } else {
         atomicAdd(&t->cnt1, 1ull);
       }
@@ -1011,17 +1148,23 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __
       atomicMin(&t->mn, hmn[i]);
       atomicMax(&t->mx, hmx[i]);
       atomicAdd(&t->cnt, (unsigned long long)hc[i]);
+    } else if (MODE == 2) {
+      atomicMin(&t->pmn, hmn[i]);
+      atomicMax(&t->pmx, hmx[i]);
     } else {
       atomicAdd(&t->cnt1, (unsigned long long)hc[i]);
     }
   }
 }
 
-__global__ __launch_bounds__(NTHREADS) void k_pf_thresh(PFP f, int nreg, unsigned long long* n_active) {
+// n[0]: active regions; n[1]: regions that get a histogram of predicted
+// values (guided, mode 0, predicted range >= min_range)
+__global__ __launch_bounds__(NTHREADS) void k_pf_thresh(PFP f, int nreg, unsigned long long* n, long long min_range,
+                                                        int st) {
   const int k = blockIdx.x * NTHREADS + threadIdx.x;
   if (k >= nreg) return;
   RegionStat& t = f.tab[k];
-  if (t.state != 1) return;
+  if (t.state != st) return;
   const long long mn = unord64(t.mn), mx = unord64(t.mx);
   if (mx - mn <= 1) {
     t.state = 2;
@@ -1031,14 +1174,105 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_thresh(PFP f, int nreg, unsigne
   long long thr = (long long)ceil(sum / (double)t.cnt);
   if (thr == mx) --thr;
   t.thr = thr;
-  atomicAdd(n_active, 1ull);
+  t.kind = 0;
+  t.state = 1;
+  t.fresh = 1;
+  atomicAdd(&n[0], 1ull);
+  if (f.pred && t.mode == 0 && unord64(t.pmx) - unord64(t.pmn) >= min_range)
+    t.hslot = (int)atomicAdd(&n[1], 1ull);
+}
+
+// histogram of the predicted values of every region with a slot (nbins bins
+// over [pmn, pmx]); an LDS table keyed by (slot, bin) folds each block's nodes
+__global__ __launch_bounds__(NTHREADS) void k_pf_hist(PFP f) {
+  constexpr int HSZ = 512;
+  __shared__ int hk[HSZ];
+  __shared__ unsigned hc[HSZ];
+  for (int i = threadIdx.x; i < HSZ; i += NTHREADS) {
+    hk[i] = -1;
+    hc[i] = 0;
+  }
+  __syncthreads();
+  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < f.N; v += (long long)gridDim.x * NTHREADS) {
+    const RegionStat& t = f.tab[f.reg[v]];
+    if (t.state != 1 || !t.fresh || t.hslot < 0) continue;
+    const long long lo = unord64(t.pmn), range = unord64(t.pmx) - lo + 1;
+    long long b = (long long)((double)(pred_r(f, v) - lo) * f.nbins / (double)range);
+    b = b < 0 ? 0 : (b >= f.nbins ? f.nbins - 1 : b);
+    const int key = t.hslot * f.nbins + (int)b;
+    int h = (int)(((unsigned)key * 2654435761u) >> 23) & (HSZ - 1);
+    int slot = -1;
+    for (int probe = 0; probe < 16; ++probe) {
+      const int cur = hk[h];
+      if (cur == key) { slot = h; break; }
+      if (cur == -1) {
+        const int old = atomicCAS(&hk[h], -1, key);
+        if (old == -1 || old == key) { slot = h; break; }
+      }
+      h = (h + 1) & (HSZ - 1);
+    }
+    if (slot >= 0) atomicAdd(&hc[slot], 1u);
+    else atomicAdd(&f.hist[key], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < HSZ; i += NTHREADS)
+    if (hk[i] >= 0 && hc[i]) atomicAdd(&f.hist[hk[i]], hc[i]);
+}
+
+// Threshold at the emptiest bin boundary of the predicted values (each side
+// holding at least min_side of the region): a cut there separates predicted
+// clusters without a large level set near the threshold.  Accepted when the two
+// bins around it hold at most gap_frac of the region.  n_pred counts them.
+__global__ __launch_bounds__(NTHREADS) void k_pf_pick(PFP f, int nreg, double gap_frac, double min_side,
+                                                      unsigned long long* n_pred) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nreg) return;
+  RegionStat& t = f.tab[k];
+  if (t.state != 1 || !t.fresh || t.hslot < 0) return;
+  const unsigned* h = f.hist + (long long)t.hslot * f.nbins;
+  const double n = (double)t.cnt;
+  unsigned long long left = h[0];
+  int best = -1;
+  double best_mass = 1e300, best_bal = 1e300;
+  for (int j = 1; j < f.nbins; ++j) {
+    const double l = (double)left, r = n - l;
+    if (l >= min_side * n && r >= min_side * n && l >= 1 && r >= 1) {
+      const double mass = (double)h[j - 1] + (double)h[j];
+      const double bal = fabs(l - r);
+      if (mass < best_mass || (mass == best_mass && bal < best_bal)) {
+        best = j;
+        best_mass = mass;
+        best_bal = bal;
+      }
+    }
+    left += h[j];
+  }
+  if (best < 0 || best_mass > gap_frac * n) return;
+  const long long lo = unord64(t.pmn), range = unord64(t.pmx) - lo + 1;
+  // first predicted value of bin `best`
+  const long long thr = lo + (long long)ceil((double)best * (double)range / (double)f.nbins);
+  const long long mn = unord64(t.mn), mx = unord64(t.mx);
+  if (thr <= mn || thr > mx) return;  // no split possible at thr
+  t.thr = thr;
+  t.kind = 1;
+  atomicAdd(n_pred, 1ull);
+}
+
+// while some region cuts at a predicted gap, the mean cuts wait
+__global__ __launch_bounds__(NTHREADS) void k_pf_defer(PFP f, int nreg) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nreg) return;
+  RegionStat& t = f.tab[k];
+  if (t.state == 1 && t.kind == 0) t.state = 3;
 }
 
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_pf_setup(PFP f, KP<T> p, int* err) {
+__global__ __launch_bounds__(NTHREADS) void k_pf_setup(PFP f, KP<T> p, int* err, int only_fresh) {
   const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
   if (v >= f.N) return;
   const RegionStat& t = f.tab[f.reg[v]];
+  // (only_fresh: running regions keep their excess; final ones drop theirs)
+  if (only_fresh && !(t.state == 1 && t.fresh) && t.state != 2) return;
   long long e = 0;
   if (t.state == 1) e = t.thr - f.r[v];
   if (sizeof(T) == 4 && (e > 0x3fffffffLL || e < -0x3fffffffLL)) atomicOr(err, 1);
@@ -1046,40 +1280,123 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_setup(PFP f, KP<T> p, int* err)
 }
 
 template <typename T>
-__global__ __launch_bounds__(NTHREADS) void k_pf_after(PFP f, KP<T> p, const int* __restrict__ reach) {
+__global__ __launch_bounds__(NTHREADS) void k_pf_after(PFP f, KP<T> p, int st) {
   const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
   if (v >= f.N) return;
   const RegionStat& t = f.tab[f.reg[v]];
-  if (t.state == 1) f.r[v] = t.thr - (long long)p.ex[v];
-  (void)reach;
+  if (t.state == st) f.r[v] = t.thr - (long long)p.ex[v];
 }
 
-__global__ __launch_bounds__(NTHREADS) void k_pf_split(PFP f, int nreg) {
+// regions in state `st` whose cut is solved; both parts continue in state `next`
+__global__ __launch_bounds__(NTHREADS) void k_pf_split(PFP f, int nreg, int st, int next) {
   const int k = blockIdx.x * NTHREADS + threadIdx.x;
   if (k >= nreg) return;
   RegionStat& t = f.tab[k];
-  if (t.state != 1) return;
+  if (t.state != st) return;
   if (t.cnt1 > 0 && t.cnt1 < t.cnt) {
-    // every region is nonempty, so ids stay below N; the test only guards memory
     const int nid = atomicAdd(f.nreg, 1);
-    if (nid >= f.N) {
+    if (nid >= f.cap) {
+      atomicOr(f.err, 1);
       t.newid = -1;
+      t.state = next;
       return;
     }
     t.newid = nid;
-    f.tab[nid].state = 1;
+    t.state = next;
+    f.tab[nid].newid = -1;
+    f.tab[nid].state = next;
+  } else if (t.kind == 1) {
+    // the predicted gap did not split the region: mean thresholds from now on
+    t.mode = 1;
+    t.newid = -1;
+    t.state = next;
+  } else if (f.collapse) {
+    // a cut at ceil(mean) that leaves the region whole: every exact value of
+    // the region lies within one quantum of the mean, so the region is final
+    // (the cuts that would only spread the leftover excess are skipped)
+    t.newid = -2;
+    t.state = 2;
   } else {
     t.newid = -1;
+    t.state = next;
   }
 }
 
+// floor(s / d) of a 128-bit two's complement s and d > 0
+__device__ inline long long floor_div128(unsigned long long hi, unsigned long long lo, long long d) {
+  const __int128 s = (__int128)(((unsigned __int128)hi << 64) | (unsigned __int128)lo);
+  long long q = (long long)floor(i128_to_double(hi, lo) / (double)d);
+  for (int it = 0; it < 16; ++it) {
+    const __int128 rem = s - (__int128)q * (__int128)d;
+    if (rem >= 0 && rem < (__int128)d) break;
+    const unsigned __int128 ur = (unsigned __int128)rem;
+    long long adj = (long long)floor(i128_to_double((unsigned long long)(ur >> 64), (unsigned long long)ur) / (double)d);
+    if (adj == 0) adj = rem < 0 ? -1 : 1;
+    q += adj;
+  }
+  return q;
+}
+
+// Output value of every node: the region's mean for the regions made final
+// by k_pf_split, else r.  (r itself stays the flow-consistent state, which a
+// merged region resumes from.)
+__global__ __launch_bounds__(NTHREADS) void k_pf_values(PFP f, long long* __restrict__ out) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const RegionStat& t = f.tab[f.reg[v]];
+  out[v] = (t.state == 2 && t.newid == -2) ? floor_div128(t.sum_hi, t.sum_lo, (long long)t.cnt) : f.r[v];
+}
+
 // source-side nodes of a split region move to the new region
-__global__ __launch_bounds__(NTHREADS) void k_pf_move(PFP f, const int* __restrict__ reach) {
+__global__ __launch_bounds__(NTHREADS) void k_pf_move(PFP f, const int* __restrict__ reach, int st) {
   const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
   if (v >= f.N) return;
   const int k = f.reg[v];
   const RegionStat& t = f.tab[k];
-  if (t.state == 1 && t.newid >= 0 && reach[v]) f.reg[v] = t.newid;
+  if (t.state == st && t.newid >= 0 && reach[v]) f.reg[v] = t.newid;
+}
+
+// ---- asynchronous divide and conquer: per-region completion -------------------
+
+// thresholds in use again (setup done): no region is fresh, no slot is held
+__global__ __launch_bounds__(NTHREADS) void k_pf_unfresh(PFP f, int nreg) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nreg) return;
+  f.tab[k].fresh = 0;
+  f.tab[k].hslot = -1;
+}
+
+// regions (state 1) with an active node (excess and a finite label)
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_pf_busy(PFP f, KP<T> p) {
+  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < f.N; v += (long long)gridDim.x * NTHREADS) {
+    if (p.ex[v] > 0 && p.h[v] < INF_H) {
+      RegionStat& t = f.tab[f.reg[v]];
+      if (t.state == 1 && !t.busy) t.busy = 1;
+    }
+  }
+}
+
+// a region with no active node after an exact relabel holds a maximum flow of
+// its cut: state 4 (n counts them); busy flags are cleared for the next relabel
+__global__ __launch_bounds__(NTHREADS) void k_pf_mark_done(PFP f, int nreg, unsigned long long* n) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nreg) return;
+  RegionStat& t = f.tab[k];
+  if (t.state != 1) return;
+  if (!t.busy) {
+    t.state = 4;
+    atomicAdd(n, 1ull);
+  }
+  t.busy = 0;
+}
+
+// S_min seeds: the excess nodes of the regions in state st
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_pf_flood_init(PFP f, KP<T> p, int* __restrict__ reach, int st) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  reach[v] = (p.ex[v] > 0 && f.tab[f.reg[v]].state == st) ? 1 : 0;
 }
 
 // zero every arc whose ends lie in different regions
@@ -1138,6 +1455,232 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_init_layers(PFP f, const long l
   f.r[v] = red0[v];
   f.reg[v] = (int)(v / plane);
   if (v < layers) f.tab[v].state = 1;
+}
+
+// ---- guided divide and conquer: an initial partition from predicted values ----
+//
+// The lattice is partitioned into regions in which neighbouring predicted
+// values differ by at most delta.  Every arc (v, u) between two regions is
+// taken as saturated from the side assumed lower (smaller prediction) to the
+// side assumed higher: r_v += c(v->u), r_u -= c(v->u), and the arc is cut.
+// The regions are then independent problems for the divide and conquer.  When
+// it is done, the assumption is checked on every cut arc (final r of the
+// higher end >= final r of the lower end + 2, i.e. x_high >= x_low for the
+// exact level values within +-1): where it holds on every arc, the optimality
+// conditions of the whole problem hold (the subgradient of w |x_v - x_u| at
+// x_v <= x_u includes the one used), so the result is its unique solution.
+// Regions joined by a failing arc are merged and solved again from their
+// original network.
+
+__device__ inline int uf_load(const int* p, int x) {
+  return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline int uf_find(int* p, int x) {
+  for (;;) {
+    const int px = uf_load(p, x);
+    if (px == x) return x;
+    const int gx = uf_load(p, px);
+    // path halving: any ancestor is a valid parent
+    if (gx != px) __hip_atomic_store(p + x, gx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = gx;
+  }
+}
+
+// links the larger root below the smaller one, so parents only decrease
+__device__ inline void uf_union(int* p, int a, int b) {
+  for (;;) {
+    a = uf_find(p, a);
+    b = uf_find(p, b);
+    if (a == b) return;
+    if (a < b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    int expected = a;
+    if (__hip_atomic_compare_exchange_strong(p + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return;
+  }
+}
+
+struct GP {
+  const float* pred;  // predicted value per node
+  const long long* val;  // output value per node (k_pf_values)
+  int* preg;          // partition region per node (0 .. nparts-1)
+  int* parent;        // union-find over nodes, later over partition regions
+  int* idmap;         // root -> compact id, later partition region -> table id
+  int* merged;        // per partition region: part of a merge
+  int* layer_bad;     // per layer: a failing arc
+  float delta;
+};
+
+__device__ inline bool assumed_lower(const float* pred, long long v, long long u) {
+  const float a = pred[v], b = pred[u];
+  return a < b || (a == b && v < u);
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_iota(int* a, long long n) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v < n) a[v] = (int)v;
+}
+
+template <int K>
+__global__ __launch_bounds__(NTHREADS) void k_part_join(PFP f, DirTab dirs, GP g) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const int x = (int)(v % f.W), y = (int)((v / f.W) % f.H);
+  const float pv = g.pred[v];
+#pragma unroll
+  for (int i = 0; i < K / 2; ++i) {
+    const int ny = y + dirs.dy[i], nx = x + dirs.dx[i];
+    if (ny < 0 || ny >= f.H || nx < 0 || nx >= f.W) continue;
+    const long long u = v + (long long)dirs.dy[i] * f.W + dirs.dx[i];
+    if (fabsf(pv - g.pred[u]) <= g.delta) uf_union(g.parent, (int)v, (int)u);
+  }
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_uf_flatten(int* parent, long long n) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v < n) parent[v] = uf_find(parent, (int)v);
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_part_ids(PFP f, GP g) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  if (g.parent[v] == (int)v) g.idmap[v] = atomicAdd(f.nreg, 1);
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_part_assign(PFP f, GP g) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const int id = g.idmap[g.parent[v]];
+  g.preg[v] = id;
+  f.reg[v] = id;
+  f.tab[id].state = 1;
+}
+
+// r = red0 + the saturated cut arcs of the partition
+template <int K, typename T>
+__global__ __launch_bounds__(NTHREADS) void k_part_network(PFP f, KP<T> p, GP g, const long long* __restrict__ red0,
+                                                           const long long* __restrict__ alpha0) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const int k = g.preg[v];
+  const int x = (int)(v % f.W), y = (int)((v / f.W) % f.H);
+  long long r = red0[v];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int ny = y + p.dirs.dy[i], nx = x + p.dirs.dx[i];
+    if (ny < 0 || ny >= f.H || nx < 0 || nx >= f.W) continue;
+    const long long u = v + (long long)p.dirs.dy[i] * f.W + p.dirs.dx[i];
+    if (g.preg[u] == k) continue;
+    if (assumed_lower(g.pred, v, u)) r += alpha0[(long long)i * f.N + v];
+    else r -= alpha0[(long long)((i + K / 2) % K) * f.N + u];
+  }
+  f.r[v] = r;
+}
+
+// Merged regions continue from their current state: every cut arc inside a
+// merged group (a partition arc or a divide-and-conquer arc) carries the flow
+// that saturated it from its lower end, so its residuals are 0 from the lower
+// end and c(lo->hi) + c(hi->lo) from the higher end; r already holds that flow.
+// The lower end of a partition arc is the assumed one, of a divide-and-conquer
+// arc the end with the smaller final r (a nested cut puts it strictly lower).
+template <int K, typename T>
+__global__ __launch_bounds__(NTHREADS) void k_part_restore(PFP f, KP<T> p, GP g, const long long* __restrict__ alpha0) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const int k = g.preg[v];
+  if (!g.merged[k]) return;
+  const int grp = g.parent[k];
+  const int rv = f.reg[v];
+  const int x = (int)(v % f.W), y = (int)((v / f.W) % f.H);
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int ny = y + p.dirs.dy[i], nx = x + p.dirs.dx[i];
+    if (ny < 0 || ny >= f.H || nx < 0 || nx >= f.W) continue;
+    const long long u = v + (long long)p.dirs.dy[i] * f.W + p.dirs.dx[i];
+    const int ku = g.preg[u];
+    if (g.parent[ku] != grp || f.reg[u] == rv) continue;
+    const bool lo = ku != k ? assumed_lower(g.pred, v, u) : g.val[v] < g.val[u];
+    p.res[(long long)i * f.N + v] =
+        lo ? (T)0 : (T)(alpha0[(long long)i * f.N + v] + alpha0[(long long)((i + K / 2) % K) * f.N + u]);
+  }
+}
+
+// failing cut arcs: union of the two partition regions (g.parent over regions)
+template <int K>
+__global__ __launch_bounds__(NTHREADS) void k_part_check(PFP f, DirTab dirs, GP g, unsigned long long* n_bad) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const int x = (int)(v % f.W), y = (int)((v / f.W) % f.H);
+  const int k = g.preg[v];
+  int bad = 0;
+#pragma unroll
+  for (int i = 0; i < K / 2; ++i) {
+    const int ny = y + dirs.dy[i], nx = x + dirs.dx[i];
+    if (ny < 0 || ny >= f.H || nx < 0 || nx >= f.W) continue;
+    const long long u = v + (long long)dirs.dy[i] * f.W + dirs.dx[i];
+    const int ku = g.preg[u];
+    if (ku == k) continue;
+    const bool lo = assumed_lower(g.pred, v, u);
+    const long long gap = lo ? g.val[u] - g.val[v] : g.val[v] - g.val[u];
+    if (gap < 2) {
+      uf_union(g.parent, k, ku);
+      ++bad;
+    }
+  }
+  if (bad) {
+    atomicAdd(n_bad, (unsigned long long)bad);
+    g.layer_bad[v / ((long long)f.H * f.W)] = 1;
+  }
+}
+
+// every region of a layer with a failing arc joins the layer's first region
+__global__ __launch_bounds__(NTHREADS) void k_part_merge_layers(PFP f, GP g) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const long long plane = (long long)f.H * f.W;
+  const long long z = v / plane;
+  if (g.layer_bad[z]) uf_union(g.parent, g.preg[v], g.preg[z * plane]);
+}
+
+__global__ __launch_bounds__(NTHREADS) void k_part_mark(GP g, int nparts) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nparts) return;
+  const int root = g.parent[k];
+  if (root != k) {
+    g.merged[k] = 1;
+    atomicOr(&g.merged[root], 1);
+  }
+}
+
+// fresh table ids for the merged regions (roots of the region union-find)
+__global__ __launch_bounds__(NTHREADS) void k_part_newids(PFP f, GP g, int nparts) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k >= nparts) return;
+  if (!g.merged[k] || g.parent[k] != k) return;
+  const int nid = atomicAdd(f.nreg, 1);
+  if (nid >= f.cap) {
+    atomicOr(f.err, 1);
+    g.idmap[k] = 0;
+    return;
+  }
+  g.idmap[k] = nid;
+  f.tab[nid].state = 1;
+}
+
+// nodes of merged regions join their group (table id from k_part_newids)
+__global__ __launch_bounds__(NTHREADS) void k_part_relabel(PFP f, GP g) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= f.N) return;
+  const int k = g.preg[v];
+  if (!g.merged[k]) return;
+  const int grp = g.parent[k];
+  g.preg[v] = grp;
+  f.reg[v] = g.idmap[grp];
 }
 
 __global__ __launch_bounds__(NTHREADS) void k_max_label(const int* __restrict__ h, long long N, long long* part) {
@@ -1222,7 +1765,9 @@ class Engine final : public MinCutEngine {
     hipFree(red0_); hipFree(alpha0_); hipFree(res_); hipFree(mbox_); hipFree(ex_); hipFree(h_);
     hipFree(mask_); hipFree(reach_); hipFree(flags_[0]); hipFree(flags_[1]); hipFree(bflags_[0]); hipFree(bflags_[1]); hipFree(dev_small_); hipFree(part_);
     hipHostFree(host_small_);
-    hipFree(pf_r_); hipFree(pf_reg_); hipFree(pf_tab_); hipFree(pf_nreg_);
+    hipFree(pf_r_); hipFree(pf_reg_); hipFree(pf_tab_); hipFree(pf_nreg_); hipFree(pf_cnt_); hipFree(pf_hist_);
+    hipFree(b_rects_); hipFree(b_blk_); hipFree(b_flows_);
+    hipFree(g_preg_); hipFree(g_parent_); hipFree(g_idmap_); hipFree(g_merged_); hipFree(g_layer_bad_); hipFree(g_val_);
     hipEventDestroy(ev_[0]);
     hipEventDestroy(ev_[1]);
     hipStreamDestroy(s_);
@@ -1248,6 +1793,7 @@ class Engine final : public MinCutEngine {
   }
 
   void reset_energies() override {
+    tv_mode_ = false;
     LF_HIP(hipMemsetAsync(red0_, 0, sizeof(long long) * N_, s_));
     LF_HIP(hipMemsetAsync(alpha0_, 0, sizeof(long long) * N_ * K, s_));
     solved = false;
@@ -1423,6 +1969,7 @@ class Engine final : public MinCutEngine {
   void pf_run(const lf_tuning& tun) override { pf_run_layers(tun, 1); }
 
   void set_tv_network(const int64_t* uf, const int64_t* pwf, int layers) override {
+    tv_mode_ = true;
     const long long plane = N_ / layers;
     long long *duf = nullptr, *dpwf = nullptr;
     LF_HIP(hipMallocAsync((void**)&duf, sizeof(long long) * plane, s_));
@@ -1444,45 +1991,16 @@ class Engine final : public MinCutEngine {
     const auto t0 = clk::now();
     begin_stats();
     load_working();
-    ensure_pf_buffers();
-    KP<T> p = params();
+    ensure_pf_buffers(N_);
     PFP pf = pf_params();
-    // one region holding every node
-    hipLaunchKernelGGL((k_fill<char>), dim3(blocks_for((long long)sizeof(RegionStat) * N_)), dim3(NTHREADS), 0, s_,
-                       reinterpret_cast<char*>(pf.tab), (char)0, (long long)sizeof(RegionStat) * N_);
+    clear_table();
+    zero_int(pf.err, 1);
     // one region per layer
     hipLaunchKernelGGL((k_pf_init_layers), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, red0_, N_ / layers, layers);
     hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(NTHREADS), 0, s_, pf.nreg, layers, 1LL);
     long long levels = 0, sections = 0;
-    for (;;) {
-      // region statistics and thresholds (setToMeanReductionPoint)
-      const int nreg = (int)std::min<long long>(read_int(pf.nreg), N_);
-      hipLaunchKernelGGL((k_pf_clear), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg);
-      hipLaunchKernelGGL((k_pf_stats_impl<0>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)nullptr);
-      zero_counter();
-      hipLaunchKernelGGL((k_pf_thresh), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, counter_ptr());
-      LF_HIP(hipGetLastError());
-      const long long active = (long long)read_counter();
-      if (active == 0) break;
-      ++levels;
-      sections += active;
-      // section min-cut of every active region at once
-      int* err = reinterpret_cast<int*>(dev_small_);
-      LF_HIP(hipMemsetAsync(err, 0, sizeof(int), s_));
-      hipLaunchKernelGGL((k_pf_setup<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, err);
-      LF_HIP(hipMemcpyAsync(host_small_, err, sizeof(int), hipMemcpyDeviceToHost, s_));
-      LF_HIP(hipStreamSynchronize(s_));
-      if (*reinterpret_cast<int*>(host_small_))
-        fail(LF_ERR_OVERFLOW, "parametric offsets exceed the capacity width; use cap_bytes=8");
-      solve_state(tun);
-      // r from the flow, split by S_min, cut the arcs between the new regions
-      hipLaunchKernelGGL((k_pf_after<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, reach_);
-      hipLaunchKernelGGL((k_pf_stats_impl<1>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)reach_);
-      hipLaunchKernelGGL((k_pf_split), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg);
-      hipLaunchKernelGGL((k_pf_move), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, (const int*)reach_);
-      hipLaunchKernelGGL((k_pf_cutarcs<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p);
-      LF_HIP(hipGetLastError());
-    }
+    dnc_loop(tun, levels, sections);
+    if (read_int(pf.err)) fail(LF_ERR_STATE, "region table overflow in the divide and conquer");
     LF_HIP(hipStreamSynchronize(s_));
     end_stats();
     stats.reserved = levels;
@@ -1493,8 +2011,291 @@ class Engine final : public MinCutEngine {
     solved = false;
   }
 
+  void pf_run_layers_guided(const lf_tuning& tun, int layers, const float* d_pred, float delta, int max_repairs,
+                            double pscale, double poff, lf_guided_stats* gs) override {
+    struct Reset {
+      GuideParams& g;
+      ~Reset() { g.pred = nullptr; }
+    } reset_guide{guide_};
+    guide_.pred = d_pred;
+    guide_.pscale = pscale;
+    guide_.poff = poff;
+    guide_.min_range = (long long)(env_double("LF_GUIDE_RANGE", 0.02) * pscale);
+    guide_.gap_frac = env_double("LF_GUIDE_GAP", 0.02);
+    guide_.min_side = env_double("LF_GUIDE_SIDE", 0.002);
+    guide_.nbins = (int)env_double("LF_GUIDE_BINS", 32);
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    lf_guided_stats g0{};
+    if (N_ >= (1LL << 31) - 1) fail(LF_ERR_UNSUPPORTED, "guided divide and conquer needs fewer than 2^31 nodes");
+    begin_stats();
+    load_working();
+    ensure_pf_buffers(2 * N_ + 1024);
+    ensure_guide_buffers(layers);
+    KP<T> p = params();
+    PFP pf = pf_params();
+    GP g = guide_params(d_pred, delta);
+    clear_table();
+    zero_int(pf.err, 1);
+    // partition: components of the arcs whose predicted values differ by <= delta
+    hipLaunchKernelGGL((k_iota), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, g.parent, N_);
+    hipLaunchKernelGGL((k_part_join<K>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p_.dirs, g);
+    hipLaunchKernelGGL((k_uf_flatten), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, g.parent, N_);
+    zero_int(pf.nreg, 1);
+    hipLaunchKernelGGL((k_part_ids), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, g);
+    hipLaunchKernelGGL((k_part_assign), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, g);
+    hipLaunchKernelGGL((k_part_network<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, g, red0_, alpha0_);
+    hipLaunchKernelGGL((k_pf_cutarcs<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p);
+    LF_HIP(hipGetLastError());
+    const int nparts = read_int(pf.nreg);
+    g0.regions = nparts;
+    const double t_part = std::chrono::duration<double>(clk::now() - t0).count();
+    long long levels = 0, sections = 0;
+    bool overflow = false;
+    for (int repair = 0;; ++repair) {
+      dnc_loop(tun, levels, sections);
+      if (read_int(pf.err)) {
+        overflow = true;
+        break;
+      }
+      // check every cut arc; failing arcs join their regions
+      hipLaunchKernelGGL((k_pf_values), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, g_val_);
+      hipLaunchKernelGGL((k_iota), dim3(blocks_for(nparts)), dim3(NTHREADS), 0, s_, g.parent, (long long)nparts);
+      zero_int(g.layer_bad, layers);
+      zero_counter();
+      hipLaunchKernelGGL((k_part_check<K>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p_.dirs, g, counter_ptr());
+      LF_HIP(hipGetLastError());
+      const long long bad = (long long)read_counter();
+      if (trace_pf_) std::fprintf(stderr, "[lf-pf] check %d: %lld failing arcs\n", repair, bad);
+      if (bad == 0) break;
+      g0.failing_arcs += bad;
+      ++g0.repairs;
+      if (repair >= max_repairs) {
+        // give up on the partition of every layer that still fails
+        hipLaunchKernelGGL((k_part_merge_layers), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, g);
+        ++g0.layers_merged;
+      }
+      hipLaunchKernelGGL((k_uf_flatten), dim3(blocks_for(nparts)), dim3(NTHREADS), 0, s_, g.parent, (long long)nparts);
+      zero_int(g.merged, nparts);
+      hipLaunchKernelGGL((k_part_mark), dim3(blocks_for(nparts)), dim3(NTHREADS), 0, s_, g, nparts);
+      hipLaunchKernelGGL((k_part_newids), dim3(blocks_for(nparts)), dim3(NTHREADS), 0, s_, pf, g, nparts);
+      hipLaunchKernelGGL((k_part_restore<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, g, alpha0_);
+      hipLaunchKernelGGL((k_part_relabel), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, g);
+      LF_HIP(hipGetLastError());
+      if (read_int(pf.err)) {
+        overflow = true;
+        break;
+      }
+    }
+    if (overflow) {
+      // region table full: solve without the partition
+      g0.fallback = 1;
+      const lf_guided_stats keep = g0;
+      guide_.pred = nullptr;
+      pf_run_layers(tun, layers);
+      if (gs) *gs = keep;
+      return;
+    }
+    LF_HIP(hipStreamSynchronize(s_));
+    end_stats();
+    stats.reserved = levels;
+    pf_levels = levels;
+    pf_sections = sections;
+    stats.seconds = std::chrono::duration<double>(clk::now() - t0).count();
+    g0.partition_seconds = t_part;
+    if (gs) *gs = g0;
+    pf_solved = true;
+    solved = false;
+  }
+
+  // Region statistics and thresholds (setToMeanReductionPoint) of the regions
+  // in state st; they become active (state 1, fresh) or final.  With
+  // predictions, regions with a gap in their predicted values cut there; with
+  // `defer`, mean cuts then wait for a later depth.  Returns the number of
+  // regions that cut (and in *predicted those at a predicted gap).
+  long long thresholds(PFP& pf, int nreg, int st, bool defer, long long* predicted) {
+    hipLaunchKernelGGL((k_pf_clear), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, st);
+    hipLaunchKernelGGL((k_pf_stats_impl<0>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)nullptr, st);
+    if (pf.pred)
+      hipLaunchKernelGGL((k_pf_stats_impl<2>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)nullptr, st);
+    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, pf_cnt_, 0ull, 4LL);
+    hipLaunchKernelGGL((k_pf_thresh), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, pf_cnt_,
+                       guide_.min_range, st);
+    LF_HIP(hipGetLastError());
+    unsigned long long cnt[4];
+    read_u64(pf_cnt_, cnt, 2);
+    long long active = (long long)cnt[0];
+    *predicted = 0;
+    if (active > 0 && cnt[1] > 0) {
+      ensure_hist((long long)cnt[1] * pf.nbins);
+      pf.hist = pf_hist_;
+      hipLaunchKernelGGL((k_fill<unsigned>), dim3(blocks_for((long long)cnt[1] * pf.nbins)), dim3(NTHREADS), 0, s_,
+                         pf_hist_, 0u, (long long)cnt[1] * pf.nbins);
+      hipLaunchKernelGGL((k_pf_hist), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf);
+      hipLaunchKernelGGL((k_pf_pick), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, guide_.gap_frac,
+                         guide_.min_side, pf_cnt_ + 2);
+      LF_HIP(hipGetLastError());
+      read_u64(pf_cnt_ + 2, cnt + 2, 1);
+      *predicted = (long long)cnt[2];
+      if (defer && *predicted > 0) {
+        hipLaunchKernelGGL((k_pf_defer), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg);
+        active = *predicted;
+      }
+    }
+    return active;
+  }
+
+  // excess t - r of the regions that cut (all nodes, or only the fresh regions)
+  void setup_excess(const PFP& pf, const KP<T>& p, int only_fresh, int nreg) {
+    int* err = reinterpret_cast<int*>(dev_small_);
+    LF_HIP(hipMemsetAsync(err, 0, sizeof(int), s_));
+    hipLaunchKernelGGL((k_pf_setup<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, err, only_fresh);
+    hipLaunchKernelGGL((k_pf_unfresh), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg);
+    LF_HIP(hipMemcpyAsync(host_small_, err, sizeof(int), hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    if (*reinterpret_cast<int*>(host_small_))
+      fail(LF_ERR_OVERFLOW, "parametric offsets exceed the capacity width; use cap_bytes=8");
+  }
+
+  // Depths of the divide and conquer until no region is active: every region
+  // of a depth is cut by one lattice min-cut (the reference's order).
+  void dnc_loop(const lf_tuning& tun, long long& levels, long long& sections) {
+    if (tv_mode_ && async_) return dnc_async(tun, levels, sections);
+    using clk = std::chrono::steady_clock;
+    KP<T> p = params();
+    PFP pf = pf_params();
+    for (;;) {
+      const int nreg = (int)std::min<long long>(read_int(pf.nreg), pf_cap_);
+      long long predicted = 0;
+      const long long active = thresholds(pf, nreg, 1, true, &predicted);
+      if (active == 0) break;
+      ++levels;
+      sections += active;
+      // section min-cut of every active region at once
+      setup_excess(pf, p, 0, nreg);
+      const lf_stats st0 = stats;
+      const auto td = clk::now();
+      solve_state(tun);
+      if (trace_pf_)
+        std::fprintf(stderr, "[lf-pf] depth %lld regions %d active %lld predicted %lld solve %.4fs cycles %lld rounds %lld bfs %lld push %.4fs\n",
+                     levels, nreg, active, predicted, std::chrono::duration<double>(clk::now() - td).count(),
+                     stats.cycles - st0.cycles, stats.rounds - st0.rounds, stats.bfs_launches - st0.bfs_launches,
+                     stats.push_seconds - st0.push_seconds);
+      // r from the flow, split by S_min, cut the arcs between the new regions
+      hipLaunchKernelGGL((k_pf_after<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, 1);
+      hipLaunchKernelGGL((k_pf_stats_impl<1>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)reach_, 1);
+      hipLaunchKernelGGL((k_pf_split), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, 1, 1);
+      hipLaunchKernelGGL((k_pf_move), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, (const int*)reach_, 1);
+      hipLaunchKernelGGL((k_pf_cutarcs<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p);
+      LF_HIP(hipGetLastError());
+    }
+  }
+
+  // Asynchronous divide and conquer: after every exact relabel, each region
+  // with no active node left holds a maximum flow of its cut and is split and
+  // given its next threshold at once, while the other regions keep pushing.
+  // The cuts are the same as in dnc_loop; only their order over regions
+  // differs, and regions never exchange flow, so the result is the same.
+  void dnc_async(const lf_tuning& tun, long long& levels, long long& sections) {
+    using clk = std::chrono::steady_clock;
+    KP<T> p = params();
+    PFP pf = pf_params();
+    int nreg = (int)std::min<long long>(read_int(pf.nreg), pf_cap_);
+    long long predicted = 0;
+    long long active = thresholds(pf, nreg, 1, false, &predicted);
+    if (active == 0) return;
+    setup_excess(pf, p, 0, nreg);
+    ++levels;
+    sections += active;
+    set_grid(tun.tile_shape >= 1 && tun.tile_shape <= 3 ? tun.tile_shape - 1 : 0);
+    p = params();
+    const bool fixed_r = tun.rounds_per_relabel > 0;
+    int R = fixed_r ? tun.rounds_per_relabel : 4;
+    bool pushed = false;
+    for (;;) {
+      const auto tb = clk::now();
+      bool exact = false;
+      const long long act = global_relabel(p, pushed, 0, &exact);
+      pushed = false;
+      // regions without active nodes are solved
+      hipLaunchKernelGGL((k_pf_busy<T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, p);
+      hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, pf_cnt_ + 3, 0ull, 1LL);
+      hipLaunchKernelGGL((k_pf_mark_done), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, pf_cnt_ + 3);
+      unsigned long long ndone = 0;
+      read_u64(pf_cnt_ + 3, &ndone, 1);
+      const double t_relabel = std::chrono::duration<double>(clk::now() - tb).count();
+      stats.bfs_seconds += t_relabel;
+      ++stats.cycles;
+      if (ndone > 0) {
+        const auto tf = clk::now();
+        // r from the flow, S_min of the solved regions, split, next thresholds
+        hipLaunchKernelGGL((k_pf_after<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, 4);
+        hipLaunchKernelGGL((k_pf_flood_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, reach_, 4);
+        run_tiled([&](const BP& q) {
+          hipLaunchKernelGGL((k_flood_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, reach_);
+          ++stats.flood_launches;
+        });
+        hipLaunchKernelGGL((k_pf_stats_impl<1>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)reach_, 4);
+        hipLaunchKernelGGL((k_pf_split), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, 4, 5);
+        hipLaunchKernelGGL((k_pf_move), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, (const int*)reach_, 5);
+        hipLaunchKernelGGL((k_pf_cutarcs<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p);
+        LF_HIP(hipGetLastError());
+        if (read_int(pf.err)) return;  // region table full: the caller falls back
+        nreg = (int)std::min<long long>(read_int(pf.nreg), pf_cap_);
+        const long long fresh = thresholds(pf, nreg, 5, false, &predicted);
+        if (fresh > 0) setup_excess(pf, p, 1, nreg);
+        ++levels;
+        sections += fresh;
+        stats.flood_seconds += std::chrono::duration<double>(clk::now() - tf).count();
+        if (trace_pf_)
+          std::fprintf(stderr, "[lf-pf] cycle %lld solved %llu new cuts %lld predicted %lld regions %d active nodes %lld\n",
+                       stats.cycles, ndone, fresh, predicted, nreg, act);
+        if (act == 0 && fresh == 0) break;  // every region final
+        continue;                           // new excesses: relabel before pushing
+      }
+      if (act == 0) break;
+      p.iters = tun.local_iters > 0 ? tun.local_iters : (act * 64 < N_ ? 16 : 4);
+      const int chunk = fixed_r ? R : 4;
+      int done_rounds = 0;
+      double t_push = 0;
+      while (done_rounds < R) {
+        const unsigned long long tiles0 = fixed_r ? 0 : read_tiles();
+        LF_HIP(hipEventRecord(ev_[0], s_));
+        const int n = std::min(chunk, R - done_rounds);
+        for (int r = 0; r < n; ++r) {
+          KP<T> q = p;
+          q.flag_in = flags_[cur_];
+          q.flag_out = flags_[cur_ ^ 1];
+          with_geo([&](auto g) {
+            using GG = decltype(g);
+            hipLaunchKernelGGL((k_push<K, GG, T>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q);
+          });
+          cur_ ^= 1;
+          ++stats.rounds;
+        }
+        LF_HIP(hipGetLastError());
+        LF_HIP(hipEventRecord(ev_[1], s_));
+        pushed = true;
+        LF_HIP(hipEventSynchronize(ev_[1]));
+        float ms = 0;
+        LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        t_push += ms * 1e-3;
+        done_rounds += n;
+        if (!fixed_r && read_tiles() == tiles0) break;  // no tile had work
+      }
+      stats.push_seconds += t_push;
+      if (!fixed_r) {
+        const double per_round = t_push / std::max(done_rounds, 1);
+        const double want = per_round > 0 ? t_relabel / per_round : 4.0;
+        R = (int)std::max(4.0, std::min(256.0, want));
+      }
+    }
+  }
+
   void pf_values_device(int64_t* d_out, hipStream_t s) override {
-    LF_HIP(hipMemcpyAsync(d_out, pf_r_, sizeof(long long) * N_, hipMemcpyDeviceToDevice, s_));
+    hipLaunchKernelGGL((k_pf_values), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf_params(),
+                       reinterpret_cast<long long*>(d_out));
+    LF_HIP(hipGetLastError());
     sync_to(s);
   }
   void pf_cut_device(int32_t* d_out, hipStream_t s) override {
@@ -1503,6 +2304,7 @@ class Engine final : public MinCutEngine {
     sync_to(s);
   }
   void set_base_network(const int64_t* red, const int64_t* alpha_lattice_major) override {
+    tv_mode_ = false;
     // host arrays: red[N], alpha[N*K] (node-major, lattice direction order)
     std::vector<long long> planes((size_t)N_ * K);
     for (long long v = 0; v < N_; ++v)
@@ -1512,6 +2314,76 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipStreamSynchronize(s_));
     solved = false;
     pf_solved = false;
+  }
+
+  // ---- batched lattices ------------------------------------------------------
+  void batch_set_layout(const std::vector<BatchRect>& rects) override {
+    if (g_.ndim != 2 || g_.D != 1) fail(LF_ERR_UNSUPPORTED, "batches are 2-D only");
+    const int n = (int)rects.size();
+    std::vector<long long> blk((size_t)n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+      const BatchRect& R = rects[i];
+      if (R.h <= 0 || R.w <= 0 || R.y0 < 0 || R.x0 < 0 || R.y0 + R.h > g_.H || R.x0 + R.w > g_.W)
+        fail(LF_ERR_BAD_ARG, "batch lattice outside the canvas");
+      blk[i + 1] = blk[i] + (R.h * R.w + BCHUNK - 1) / BCHUNK;
+    }
+    if (blk[n] > 0x7fffffffLL) fail(LF_ERR_BAD_ARG, "batch too large");
+    hipFree(b_rects_);
+    hipFree(b_blk_);
+    hipFree(b_flows_);
+    b_rects_ = nullptr; b_blk_ = nullptr; b_flows_ = nullptr;
+    LF_HIP(hipMalloc(&b_rects_, sizeof(BatchRect) * std::max(n, 1)));
+    LF_HIP(hipMalloc(&b_blk_, sizeof(long long) * (n + 1)));
+    LF_HIP(hipMalloc(&b_flows_, sizeof(unsigned long long) * std::max(n, 1)));
+    LF_HIP(hipMemcpyAsync(b_rects_, rects.data(), sizeof(BatchRect) * n, hipMemcpyHostToDevice, s_));
+    LF_HIP(hipMemcpyAsync(b_blk_, blk.data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    b_rects_host_ = rects;
+    b_nblocks_ = blk[n];
+  }
+
+  void batch_add_energies_device(int i, const int64_t* du, const int64_t* db, hipStream_t s) override {
+    if (i < 0 || i >= (int)b_rects_host_.size()) fail(LF_ERR_OUT_OF_RANGE, "batch lattice index out of range");
+    if (s && s != s_) {
+      hipEvent_t ev;
+      LF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      LF_HIP(hipEventRecord(ev, s));
+      LF_HIP(hipStreamWaitEvent(s_, ev, 0));
+      LF_HIP(hipEventDestroy(ev));
+    }
+    const BatchRect& R = b_rects_host_[i];
+    int* err = reinterpret_cast<int*>(dev_small_);
+    LF_HIP(hipMemsetAsync(err, 0, sizeof(int), s_));
+    hipLaunchKernelGGL((k_batch_reparam<K>), dim3(blocks_for(R.h * R.w)), dim3(NTHREADS), 0, s_, R, p_.W, N_,
+                       p_.dirs, reinterpret_cast<const long long*>(du), reinterpret_cast<const long long*>(db),
+                       red0_, alpha0_, err);
+    LF_HIP(hipGetLastError());
+    LF_HIP(hipMemcpyAsync(host_small_, err, sizeof(int), hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    if (reinterpret_cast<int*>(host_small_)[0] & 1)
+      fail(LF_ERR_NONSUBMODULAR, "pairwise term with E00 + E11 > E01 + E10 (not submodular)");
+    solved = false;
+  }
+
+  void batch_flows(int64_t* out) override {
+    const int n = (int)b_rects_host_.size();
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(blocks_for(n)), dim3(NTHREADS), 0, s_, b_flows_, 0ull,
+                       (long long)n);
+    hipLaunchKernelGGL((k_batch_flow<T>), dim3((unsigned)b_nblocks_), dim3(NTHREADS), 0, s_, params(), red0_, b_rects_,
+                       b_blk_, n, b_flows_);
+    LF_HIP(hipGetLastError());
+    LF_HIP(hipMemcpyAsync(out, b_flows_, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+  }
+
+  void batch_cut_device(int32_t* d_out, int which, hipStream_t s) override {
+    const int n = (int)b_rects_host_.size();
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_batch_cut<T>), dim3((unsigned)b_nblocks_), dim3(NTHREADS), 0, s_, params(), reach_, which,
+                       b_rects_, b_blk_, n, d_out);
+    LF_HIP(hipGetLastError());
+    sync_to(s);
   }
 
   void cut_device(int32_t* d_out, int which, hipStream_t s) override {
@@ -1564,17 +2436,76 @@ class Engine final : public MinCutEngine {
       LF_HIP(hipStreamSynchronize(s_));
     }
   }
-  void ensure_pf_buffers() {
-    if (pf_r_) return;
-    LF_HIP(hipMalloc(&pf_r_, sizeof(long long) * N_));
-    LF_HIP(hipMalloc(&pf_reg_, sizeof(int) * N_));
-    LF_HIP(hipMalloc(&pf_tab_, sizeof(RegionStat) * N_));
-    LF_HIP(hipMalloc(&pf_nreg_, sizeof(int) * 2));
+  void ensure_pf_buffers(long long cap) {
+    if (!pf_r_) {
+      LF_HIP(hipMalloc(&pf_cnt_, sizeof(unsigned long long) * 4));
+      LF_HIP(hipMalloc(&pf_r_, sizeof(long long) * N_));
+      LF_HIP(hipMalloc(&pf_reg_, sizeof(int) * N_));
+      LF_HIP(hipMalloc(&pf_nreg_, sizeof(int) * 4));
+    }
+    if (cap > pf_cap_) {
+      hipFree(pf_tab_);
+      pf_tab_ = nullptr;
+      LF_HIP(hipMalloc(&pf_tab_, sizeof(RegionStat) * cap));
+      pf_cap_ = cap;
+    }
+  }
+  void ensure_hist(long long n) {
+    if (n <= pf_hist_cap_) return;
+    hipFree(pf_hist_);
+    pf_hist_ = nullptr;
+    pf_hist_cap_ = std::max(n, 2 * pf_hist_cap_);
+    LF_HIP(hipMalloc(&pf_hist_, sizeof(unsigned) * pf_hist_cap_));
+  }
+  void read_u64(const unsigned long long* d, unsigned long long* h, int n) {
+    LF_HIP(hipMemcpyAsync(host_small_, d, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    std::memcpy(h, host_small_, sizeof(unsigned long long) * n);
+  }
+  static double env_double(const char* name, double dflt) {
+    const char* v = std::getenv(name);
+    return v ? std::atof(v) : dflt;
+  }
+  void ensure_guide_buffers(int layers) {
+    if (!g_preg_) {
+      LF_HIP(hipMalloc(&g_preg_, sizeof(int) * N_));
+      LF_HIP(hipMalloc(&g_parent_, sizeof(int) * N_));
+      LF_HIP(hipMalloc(&g_idmap_, sizeof(int) * N_));
+      LF_HIP(hipMalloc(&g_merged_, sizeof(int) * N_));
+      LF_HIP(hipMalloc(&g_val_, sizeof(long long) * N_));
+    }
+    if (layers > g_layers_) {
+      hipFree(g_layer_bad_);
+      g_layer_bad_ = nullptr;
+      LF_HIP(hipMalloc(&g_layer_bad_, sizeof(int) * layers));
+      g_layers_ = layers;
+    }
+  }
+  GP guide_params(const float* pred, float delta) const {
+    GP g;
+    g.pred = pred; g.preg = g_preg_; g.parent = g_parent_; g.idmap = g_idmap_; g.merged = g_merged_;
+    g.layer_bad = g_layer_bad_; g.delta = delta; g.val = g_val_;
+    return g;
+  }
+  void clear_table() {
+    static_assert(sizeof(RegionStat) % 8 == 0, "RegionStat is cleared as 64-bit words");
+    const long long words = (long long)(sizeof(RegionStat) / 8) * pf_cap_;
+    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(blocks_for(words)), dim3(NTHREADS), 0, s_,
+                       reinterpret_cast<unsigned long long*>(pf_tab_), 0ull, words);
+  }
+  void zero_int(int* a, long long n) {
+    hipLaunchKernelGGL((k_fill<int>), dim3(blocks_for(n)), dim3(NTHREADS), 0, s_, a, 0, n);
   }
   PFP pf_params() const {
     PFP f;
-    f.r = pf_r_; f.reg = pf_reg_; f.tab = pf_tab_; f.nreg = pf_nreg_;
-    f.N = N_; f.D = p_.D; f.H = p_.H; f.W = p_.W;
+    f.r = pf_r_; f.reg = pf_reg_; f.tab = pf_tab_; f.nreg = pf_nreg_; f.err = pf_nreg_ + 2;
+    f.N = N_; f.cap = pf_cap_; f.D = p_.D; f.H = p_.H; f.W = p_.W;
+    f.collapse = tv_mode_ ? 1 : 0;
+    f.pred = guide_.pred;
+    f.pscale = guide_.pscale;
+    f.poff = guide_.poff;
+    f.hist = pf_hist_;
+    f.nbins = guide_.nbins;
     return f;
   }
   unsigned long long read_counter() {
@@ -1668,10 +2599,39 @@ class Engine final : public MinCutEngine {
   hipEvent_t ev_[2] = {nullptr, nullptr};
   unsigned long long max_red_ = 0, max_alpha_ = 0;
   const bool trace_ = std::getenv("LF_TRACE") != nullptr;
+  const bool trace_pf_ = std::getenv("LF_TRACE_PF") != nullptr;
+  bool tv_mode_ = false;  // TV network: regions that a mean cut leaves whole are final
+  // LF_PF_ASYNC selects the asynchronous divide and conquer for TV networks
+  // (same result; on the 1024^2 x 32-lambda path it measured slower than the
+  // depth-by-depth order, so that order is the default)
+  const bool async_ = std::getenv("LF_PF_ASYNC") != nullptr;
   long long* pf_r_ = nullptr;
   int* pf_reg_ = nullptr;
   RegionStat* pf_tab_ = nullptr;
   int* pf_nreg_ = nullptr;
+  long long pf_cap_ = 0;
+  unsigned long long* pf_cnt_ = nullptr;
+  unsigned* pf_hist_ = nullptr;
+  long long pf_hist_cap_ = 0;
+  struct GuideParams {
+    const float* pred = nullptr;
+    double pscale = 0, poff = 0;
+    long long min_range = 0;
+    double gap_frac = 0, min_side = 0;
+    int nbins = 32;
+  } guide_;
+  int* g_preg_ = nullptr;
+  int* g_parent_ = nullptr;
+  int* g_idmap_ = nullptr;
+  int* g_merged_ = nullptr;
+  long long* g_val_ = nullptr;
+  int* g_layer_bad_ = nullptr;
+  int g_layers_ = 0;
+  std::vector<BatchRect> b_rects_host_;
+  BatchRect* b_rects_ = nullptr;
+  long long* b_blk_ = nullptr;
+  unsigned long long* b_flows_ = nullptr;
+  long long b_nblocks_ = 0;
 };
 
 template <int K, class G0, class G1, class G2, class B>

@@ -248,6 +248,134 @@ class LatticeFlow(object):
         return out.reshape(*self._dimensions)
 
 
+class LatticeFlowBatch(object):
+    """Many independent 2-D lattices of one kernel, solved by one device min-cut.
+
+    The reference solves each lattice with its own ``LatticeFlow`` object
+    (pyx:82-302); a batch gives the same per-lattice results -- ``flows()[i]``
+    and ``getCuts()[i]`` equal ``flow()`` and ``getCut()`` of lattice i solved
+    alone.  Energies are added per lattice in that lattice's own layout
+    (``edges(i)`` is its edge order).
+    """
+
+    def __init__(self, kernel, shapes, numerical_precision=1e-6, cap_bytes=8):
+        dim = getKernelDimension(kernel)
+        if dim != 2:
+            raise ValueError("Kernel '%s' is %d-dimensional; batches take 2-D kernels." % (kernel, dim))
+        self._shapes = [tuple(int(d) for d in s) for s in shapes]
+        for s in self._shapes:
+            if len(s) != 2:
+                raise ValueError("every lattice of a batch is 2-D; got shape %s" % (s,))
+        arr = np.ascontiguousarray(np.array(self._shapes, dtype=np.int64).reshape(-1))
+        h = ctypes.c_void_p()
+        check(lib().lf_batch_create(ctypes.byref(h), kernel.encode(), len(self._shapes), arr.ctypes.data,
+                                    int(cap_bytes)))
+        self._h = h
+        self._kernel = kernel
+        self.numerical_precision_factor = 1.0 / numerical_precision
+        self._kp = lib().lf_n_filling_edges(h)
+        self._offsets = np.concatenate([[0], np.cumsum([a * b for a, b in self._shapes])]).astype(np.int64)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().lf_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __len__(self):
+        return len(self._shapes)
+
+    @property
+    def shapes(self):
+        return list(self._shapes)
+
+    @property
+    def size(self):
+        return int(self._offsets[-1])
+
+    def canvas(self):
+        rc = np.empty(2, np.int64)
+        check(lib().lf_batch_canvas(self._h, rc.ctypes.data))
+        return int(rc[0]), int(rc[1])
+
+    def edges(self, i):
+        e = np.empty((self._kp, 2), dtype=np.int32)
+        check(lib().lf_batch_edge_deltas(self._h, int(i), e.ctypes.data))
+        return e
+
+    def addEnergy(self, i, unary_energy, pairwise_energy):
+        nf = self.numerical_precision_factor
+        self.addEnergyInt(i, np.round(nf * np.asarray(unary_energy)), np.round(nf * np.asarray(pairwise_energy)))
+
+    def addEnergyInt(self, i, unary_energy, pairwise_energy):
+        i = int(i)
+        if not 0 <= i < len(self._shapes):
+            raise IndexError("batch lattice index %d out of range" % i)
+        n = self._shapes[i][0] * self._shapes[i][1]
+        u = np.ascontiguousarray(unary_energy, dtype=np.int64)
+        p = np.ascontiguousarray(pairwise_energy, dtype=np.int64)
+        if u.size != 2 * n:
+            raise ValueError("unary energy of lattice %d must hold %d x 2 values" % (i, n))
+        if p.size != n * self._kp * 4:
+            raise ValueError("pairwise energy of lattice %d must hold %d x %d x 4 values" % (i, n, self._kp))
+        check(lib().lf_batch_add_energy_potentials(self._h, i, u.ctypes.data, p.ctypes.data))
+
+    def addEnergyDevice(self, i, unary_ptr, pairwise_ptr, stream=None):
+        check(lib().lf_batch_add_energy_potentials_device(self._h, int(i), ctypes.c_void_p(unary_ptr),
+                                                          ctypes.c_void_p(pairwise_ptr),
+                                                          ctypes.c_void_p(stream or 0)))
+
+    def resetEnergy(self):
+        check(lib().lf_reset_energies(self._h))
+
+    def setTuning(self, rounds_per_relabel=0, local_iters=0, max_cycles=0, tile_shape=0):
+        t = _lib.lf_tuning(int(rounds_per_relabel), int(local_iters), int(max_cycles), int(tile_shape))
+        check(lib().lf_set_tuning(self._h, ctypes.byref(t)))
+
+    def getTuning(self):
+        t = _lib.lf_tuning()
+        check(lib().lf_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, reps=1):
+        t = _lib.lf_tuning()
+        check(lib().lf_autotune(self._h, int(reps), ctypes.byref(t)))
+        return t.as_dict()
+
+    def run(self):
+        check(lib().lf_run(self._h))
+
+    def runTime(self):
+        t = ctypes.c_double()
+        check(lib().lf_run_time(self._h, ctypes.byref(t)))
+        return t.value
+
+    def stats(self):
+        s = _lib.lf_stats()
+        check(lib().lf_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def flows(self):
+        """Raw (doubled-unit) max-flow value of every lattice of the last run."""
+        out = np.empty(len(self._shapes), dtype=np.int64)
+        check(lib().lf_batch_get_flows(self._h, out.ctypes.data))
+        return out
+
+    def getCutsFlat(self, which="smin"):
+        sel = {"smin": _lib.LF_CUT_SMIN, "smax": _lib.LF_CUT_SMAX}[which]
+        out = np.empty(self.size, dtype=np.int32)
+        check(lib().lf_get_cut(self._h, out.ctypes.data, sel))
+        return out
+
+    def getCuts(self, which="smin"):
+        """Cut of every lattice, a list of int32 arrays of each lattice's shape."""
+        flat = self.getCutsFlat(which)
+        return [flat[self._offsets[i]:self._offsets[i + 1]].reshape(s) for i, s in enumerate(self._shapes)]
+
+
 def _tv_single(X, lam, kernel):
     X = np.ascontiguousarray(X, dtype=np.float64)
     if X.ndim != 2:
@@ -280,6 +408,25 @@ def _tv_path(X, lambdas, kernel):
     check(lib().lf_tv_path(kernel.encode(), nx, ny, X.ctypes.data, lam.size, lam.ctypes.data,
                            R.ctypes.data, ctypes.byref(st)))
     return R, st.as_dict()
+
+
+def _tv_path_guided(X, lambdas, kernel, pred, delta, max_repairs=8):
+    """Exact TV path from an initial partition by the device predictions ``pred``
+    (a float32 CUDA tensor of shape (len(lambdas),) + X.shape).  Same output as
+    ``_tv_path``; returns (R, stats, guided_stats)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    lam = np.ascontiguousarray(lambdas, dtype=np.float64).reshape(-1)
+    nx, ny = X.shape
+    if tuple(pred.shape) != (lam.size, nx, ny) or str(pred.dtype) != "torch.float32" or not pred.is_cuda:
+        raise ValueError("pred must be a float32 device tensor of shape %s" % ((lam.size, nx, ny),))
+    pred = pred.contiguous()
+    R = np.empty((lam.size, nx, ny), np.float64)
+    st = _lib.lf_stats()
+    gs = _lib.lf_guided_stats()
+    check(lib().lf_tv_path_guided(kernel.encode(), nx, ny, X.ctypes.data, lam.size, lam.ctypes.data,
+                                  ctypes.c_void_p(pred.data_ptr()), float(delta), int(max_repairs),
+                                  R.ctypes.data, ctypes.byref(st), ctypes.byref(gs)))
+    return R, st.as_dict(), gs.as_dict()
 
 
 def calculate2dTVPath(X, lambdas, kernel="Star2d_16", return_stats=False):

@@ -18,6 +18,8 @@ capacity -> energy mapping is E0 = c_s, E1 = c_t, E01 = E10 = w, E00 = E11 = 0.
                      c_s = round(800 max(v - 0.5, 0)), c_t = round(800 max(0.5 - v, 0)).
 * ``batch_sides``    side lengths of the batched workload (log-uniform 64..1024,
                      aspect ratio in [0.5, 2]).
+* ``batch_instances`` (shape, seed) of every lattice of the batched workload;
+                     each lattice is ``seg8`` with its own seed.
 """
 from __future__ import annotations
 
@@ -158,6 +160,12 @@ def batch_sides(seed=4, count=512, lo=64, hi=1024):
         w = int(round(s / math.sqrt(a)))
         out.append((max(2, h), max(2, w)))
     return out
+
+
+def batch_instances(seed=4, count=512, lo=64, hi=1024):
+    """The batched workload: ``count`` lattices with ``batch_sides`` shapes, lattice
+    i generated by ``seg8(seed=10000 * seed + i, shape)`` (its own seed)."""
+    return [(shape, 10000 * seed + i) for i, shape in enumerate(batch_sides(seed, count, lo, hi))]
 
 
 def n_edges(kernel_offsets, shape):

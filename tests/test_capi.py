@@ -77,3 +77,18 @@ def test_last_error_is_thread_local_string():
     assert lib().lf_kernel_dimension(b"nope") == -1
     assert b"nope" in lib().lf_last_error()
     assert lib().lf_is_valid_kernel(None) == 0
+
+
+def test_batch_create_rejects_bad_arguments():
+    """lf_batch_create validates its arguments before touching the device."""
+    from latticeflow_amd._lib import lib, LF_ERR_BAD_NAME, LF_ERR_UNSUPPORTED, LF_ERR_BAD_ARG
+    h = ctypes.c_void_p()
+    shapes = (ctypes.c_int64 * 4)(10, 20, 30, 40)
+    assert lib().lf_batch_create(ctypes.byref(h), b"Nope2d_8", 2, shapes, 4) == LF_ERR_BAD_NAME
+    assert lib().lf_batch_create(ctypes.byref(h), b"Full3d_6", 2, shapes, 4) == LF_ERR_UNSUPPORTED
+    assert b"2-D" in lib().lf_last_error()
+    assert lib().lf_batch_create(ctypes.byref(h), b"Full2d_8", 0, shapes, 4) == LF_ERR_BAD_ARG
+    assert lib().lf_batch_create(ctypes.byref(h), b"Full2d_8", 2, shapes, 3) == LF_ERR_BAD_ARG
+    bad = (ctypes.c_int64 * 4)(10, 0, 30, 40)
+    assert lib().lf_batch_create(ctypes.byref(h), b"Full2d_8", 2, bad, 4) == LF_ERR_BAD_ARG
+    assert not h.value
