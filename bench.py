@@ -265,23 +265,34 @@ def bench_batch(a):
 
 def bench_tvpath(a):
     """Exact TV regularization path: 1024x1024 Voronoi image, Full2d_8, 32 lambdas
-    log-spaced in [0.01, 2], all lambdas solved together on the device (plain
-    divide and conquer, threshold ceil(mean))."""
+    log-spaced in [0.01, 2], all lambdas solved together on the device.  Plain:
+    divide and conquer with the mean threshold; learned: thresholds from the
+    shipped net's predictions (same output).  value = plain path time."""
     import numpy as np
-    import torch  # noqa: F401  (initialises the device like the main bench)
+    import torch
+    torch.zeros(1, device="cuda")
     from latticeflow_amd import workloads as W
     from latticeflow_amd.pylatticeflow import _tv_path
+    from latticeflow_amd.tvnet import TVNet, tv_path_learned
     shape = tuple(int(x) for x in a.shape.split("x")) if a.shape else (1024, 1024)
     f = W.tv_voronoi(seed=5, shape=shape)
     lams = W.tv_lambdas()
+    net = TVNet.load()
     for _ in range(a.warmup):
         _tv_path(f, lams, "Full2d_8")
-    times = []
+    times, ltimes = [], []
+    R = Rl = st = gs = None
     for _ in range(max(a.steps, 1)):
         t0 = time.perf_counter()
         R, st = _tv_path(f, lams, "Full2d_8")
         times.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        Rl, stl, gs = tv_path_learned(f, lams, "Full2d_8", net=net)
+        torch.cuda.synchronize()
+        ltimes.append(time.perf_counter() - t0)
     dt = sum(times) / len(times)
+    dl = sum(ltimes) / len(ltimes)
+    same = float(np.abs(Rl - R).max() / np.abs(R).max())
     cpu = None
     parity = None
     if not a.no_cpu_baseline:
@@ -305,7 +316,10 @@ def bench_tvpath(a):
         "vs_baseline": None, "dtype": "int64", "data": "synthetic",
         "config": {"workload": "tvpath %s Full2d_8 Voronoi(256 cells)+N(0,0.1), 32 lambdas log[0.01,2]"
                    % "x".join(map(str, shape)), "levels_max": st["reserved"], "cycles": st["cycles"],
-                   "rounds": st["rounds"], "bfs_launches": st["bfs_launches"], "parity_vs_cpu": parity},
+                   "rounds": st["rounds"], "bfs_launches": st["bfs_launches"], "parity_vs_cpu": parity,
+                   "learned": {"seconds": round(dl, 3), "speedup_vs_plain": round(dt / dl, 3),
+                               "max_rel_diff_vs_plain": same, "guided": gs,
+                               "net": "3-layer 3x3 conv, 16 channels, bf16 MFMA (latticeflow_amd/data/tvnet_params.npy)"}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -204,6 +204,28 @@ int lf_tv_path_guided(const char *kernel, int64_t nx, int64_t ny, const double *
                       const double *lambdas, const float *d_pred, double delta, int max_repairs, double *R,
                       lf_stats *stats, lf_guided_stats *gstats);
 
+/* ---- learned TV predictor ---------------------------------------------------
+ * No reference counterpart: a 3-layer 3x3 convolutional net (16 channels, BF16
+ * matrix-core operands, FP32 accumulation) that predicts the TV output of an
+ * image at a lambda, used only to build d_pred for lf_tv_path_guided.  All
+ * pointers are device memory; calls are ordered on `stream`.  Inputs are n
+ * images of H x W: f[n*H*W] = (image - M) / S (the TV setup's mean and scale),
+ * lam[n] = log(S^2 lambda); pred / target are in the same units as f.
+ * params holds lf_tvnet_param_count() floats; work is scratch memory of at
+ * least lf_tvnet_work_bytes(n, H, W) bytes. */
+int lf_tvnet_param_count(void);
+size_t lf_tvnet_work_bytes(int n, int64_t H, int64_t W);
+int lf_tvnet_forward(const float *params, int n, int64_t H, int64_t W, const float *f, const float *lam,
+                     float *pred, void *work, size_t work_size, void *stream);
+/* Gradient of the mean squared error against target: grad[param_count],
+ * loss[1] (both overwritten). */
+int lf_tvnet_gradients(const float *params, int n, int64_t H, int64_t W, const float *f, const float *lam,
+                       const float *target, float *grad, float *loss, void *work, size_t work_size,
+                       void *stream);
+/* One Adam update (beta1 0.9, beta2 0.999, eps 1e-8) of params from grad;
+ * m and v are the moment buffers (zero before step 1). */
+int lf_tvnet_adam(float *params, const float *grad, float *m, float *v, int step, float lr, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -744,4 +744,34 @@ int lf_tv_path_guided(const char* kernel, int64_t nx, int64_t ny, const double* 
   });
 }
 
+// ---- learned TV predictor ------------------------------------------------------
+
+int lf_tvnet_param_count(void) { return tvnet_param_count(); }
+
+size_t lf_tvnet_work_bytes(int n, int64_t H, int64_t W) { return tvnet_work_bytes(n, H, W); }
+
+int lf_tvnet_forward(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam, float* pred,
+                     void* work, size_t work_size, void* stream) {
+  return guarded([&] {
+    if (!params || !f || !lam || !pred || !work) fail(LF_ERR_BAD_ARG, "null argument");
+    tvnet_forward(params, n, H, W, f, lam, pred, work, work_size, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int lf_tvnet_gradients(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam,
+                       const float* target, float* grad, float* loss, void* work, size_t work_size, void* stream) {
+  return guarded([&] {
+    if (!params || !f || !lam || !target || !grad || !loss || !work) fail(LF_ERR_BAD_ARG, "null argument");
+    tvnet_gradients(params, n, H, W, f, lam, target, grad, loss, work, work_size,
+                    reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int lf_tvnet_adam(float* params, const float* grad, float* m, float* v, int step, float lr, void* stream) {
+  return guarded([&] {
+    if (!params || !grad || !m || !v) fail(LF_ERR_BAD_ARG, "null argument");
+    tvnet_adam(params, grad, m, v, step, lr, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
 }  // extern "C"

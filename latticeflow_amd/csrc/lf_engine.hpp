@@ -96,4 +96,14 @@ class MinCutEngine {
 
 std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_bytes);
 
+// Learned TV predictor (lf_tvnet.hip): parameter count, workspace size,
+// forward pass, loss gradients and one Adam update, all on device pointers.
+int tvnet_param_count();
+size_t tvnet_work_bytes(int n, int64_t H, int64_t W);
+void tvnet_forward(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam, float* pred,
+                   void* work, size_t work_size, hipStream_t s);
+void tvnet_gradients(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam,
+                     const float* target, float* grad, float* loss, void* work, size_t work_size, hipStream_t s);
+void tvnet_adam(float* params, const float* grad, float* m, float* v, int step, float lr, hipStream_t s);
+
 }  // namespace lfamd

@@ -2020,10 +2020,14 @@ class Engine final : public MinCutEngine {
     guide_.pred = d_pred;
     guide_.pscale = pscale;
     guide_.poff = poff;
+    // threshold choice: the emptiest of 64 bins of the predicted values with at
+    // least a quarter of the region on each side, when the two bins around it
+    // hold at most 5% of the region (LF_GUIDE_* override these for studies)
     guide_.min_range = (long long)(env_double("LF_GUIDE_RANGE", 0.02) * pscale);
-    guide_.gap_frac = env_double("LF_GUIDE_GAP", 0.02);
-    guide_.min_side = env_double("LF_GUIDE_SIDE", 0.002);
-    guide_.nbins = (int)env_double("LF_GUIDE_BINS", 32);
+    guide_.gap_frac = env_double("LF_GUIDE_GAP", 0.05);
+    guide_.min_side = env_double("LF_GUIDE_SIDE", 0.25);
+    guide_.nbins = (int)env_double("LF_GUIDE_BINS", 64);
+    guide_.defer = env_double("LF_GUIDE_DEFER", 0) != 0;
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     lf_guided_stats g0{};
@@ -2167,7 +2171,7 @@ class Engine final : public MinCutEngine {
     for (;;) {
       const int nreg = (int)std::min<long long>(read_int(pf.nreg), pf_cap_);
       long long predicted = 0;
-      const long long active = thresholds(pf, nreg, 1, true, &predicted);
+      const long long active = thresholds(pf, nreg, 1, guide_.defer, &predicted);
       if (active == 0) break;
       ++levels;
       sections += active;
@@ -2619,6 +2623,7 @@ class Engine final : public MinCutEngine {
     long long min_range = 0;
     double gap_frac = 0, min_side = 0;
     int nbins = 32;
+    bool defer = false;
   } guide_;
   int* g_preg_ = nullptr;
   int* g_parent_ = nullptr;
