@@ -49,7 +49,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device-to-device 5 times (calibrates the FETCH_SIZE / WRITE_SIZE "
+                         "counters against a kernel of known traffic)")
     a = ap.parse_args()
+    if a.copy_probe:
+        return copy_probe()
     if a.workload == "tvpath":
         return bench_tvpath(a)
     if a.workload == "batch512":
@@ -115,7 +120,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic("%s %s %s" % (a.workload, "x".join(map(str, shape)), w["kernel"]), tuning),
         "avg_launch_us": round(push_s / max(launches, 1) * 1e6, 2),
         "bytes_per_node": st["bytes_per_node"],
         "nodes_per_launch": round(tiles * st["tile_nodes"] / max(launches, 1), 1),
@@ -166,6 +171,32 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(workload, tuning):
+    """HBM bytes per k_push launch from the calibrated FETCH_SIZE / WRITE_SIZE
+    counter runs recorded in profiles/traffic.json, when they were taken on
+    this workload at this tuning; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        rec = json.load(open(path)).get(workload)
+    except (OSError, ValueError):
+        return None
+    if not rec or any(tuning.get(k) != v for k, v in rec.get("tuning", {}).items()):
+        return None
+    return rec["bytes_per_launch"]
+
+
+def copy_probe():
+    import torch
+    n = 1 << 28  # 2^28 floats = 1 GiB each way
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}), flush=True)
 
 
 def bench_batch(a):

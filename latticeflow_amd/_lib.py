@@ -126,6 +126,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("liblatticeflow_amd.so is not built; run __graft_entry__.build()")
+        # PyTorch (device memory and streams for the Python host) ships its own
+        # HIP runtime; load and initialise it first so that the library and
+        # PyTorch share one runtime in this process.
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIG.items():
             f = getattr(L, name)
