@@ -756,6 +756,302 @@ __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restr
   if (__syncthreads_or(border) && threadIdx.x == 0) flag_around(p, tile);
 }
 
+// ---- wave-tile exact passes (2D lattices, 3x3 neighbourhoods) ------------------
+//
+// A 64 x 64 tile is one wavefront: lane y holds row y of the tile as 64-bit
+// words, bit x = column x.  The residual arcs are bit planes, one per
+// neighbour slot s = 3 (dy + 1) + (dx + 1) (the node itself, s = 4, has none),
+// stored tile-major so that the 64 words of one tile and plane are contiguous;
+// plane 8 holds the sink-capable nodes.  A BFS level is then a handful of
+// word operations per lane (shift the frontier of the row and of the rows
+// above and below, AND with the arc planes), with no block barrier at all,
+// and a flood closes each row with one carry-propagating add.
+
+constexpr int WT = 64;      // side of a wave tile
+constexpr int WPLANES = 9;  // 8 neighbour slots + sinks
+
+__host__ __device__ constexpr int slot_plane(int dy, int dx) {
+  return (dy + 1) * 3 + (dx + 1) < 4 ? (dy + 1) * 3 + (dx + 1) : (dy + 1) * 3 + (dx + 1) - 1;
+}
+
+struct WP {
+  int D, H, W, ntx, nty;
+  long long ps;  // words per plane = D * nty * ntx * 64
+};
+
+__device__ inline long long widx(const WP& w, int pl, int z, int ty, int tx, int lane) {
+  return (long long)pl * w.ps + ((((long long)z * w.nty + ty) * w.ntx + tx) << 6) + lane;
+}
+
+// Bit planes of the residual arcs and sinks from the working state, and the
+// BFS start labels (0 on sink-capable nodes, INF_H elsewhere).  One wave per
+// 64-node word of a row; the word is the wave's ballot.  Plane words of absent
+// directions and of rows past the lattice stay zero from allocation.
+template <int K, typename T>
+__global__ __launch_bounds__(NTHREADS) void k_bits_init(KP<T> p, WP w, unsigned long long* __restrict__ planes) {
+  const long long wave = ((long long)blockIdx.x * NTHREADS + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= (long long)p.D * p.H * w.ntx) return;  // uniform over the wave
+  const int tx = (int)(wave % w.ntx);
+  const long long r = wave / w.ntx;  // row over all layers
+  const int y = (int)(r % p.H), z = (int)(r / p.H);
+  const int x = tx * WT + lane;
+  const bool in = x < p.W;
+  const long long v = r * p.W + x;
+  const long long base = widx(w, 0, z, y >> 6, tx, y & 63);
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const unsigned long long b = __ballot(in && p.res[(long long)i * p.N + v] > 0);
+    if (lane == 0) planes[base + (long long)slot_plane(p.dirs.dy[i], p.dirs.dx[i]) * w.ps] = b;
+  }
+  const T e = in ? p.ex[v] : (T)0;
+  const unsigned long long s = __ballot(e < 0);
+  if (lane == 0) planes[base + 8LL * w.ps] = s;
+  if (in) p.h[v] = e < 0 ? 0 : INF_H;
+}
+
+__device__ inline int wave_min_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int u = __shfl_xor(v, o, 64);
+    v = u < v ? u : v;
+  }
+  return v;
+}
+
+// Flag the 8 neighbouring tiles of the same layer and count a border change
+// (2D neighbourhoods never cross layers).
+__device__ inline void flag_around2d(const BP& p, int tile) {
+  atomicAdd(p.counter, 1ull);
+  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
+  for (int b = -1; b <= 1; ++b)
+    for (int e = -1; e <= 1; ++e) {
+      const int y = ty + b, x = tx + e;
+      if ((b | e) == 0 || y < 0 || y >= p.nty || x < 0 || x >= p.ntx) continue;
+      p.flag_out[(tz * p.nty + y) * p.ntx + x] = 1;
+    }
+}
+
+// One wave tile of the exact distance labelling (same fixpoint as k_bfs_tile):
+// the tile's labels are recomputed level by level from its sinks and from the
+// current labels of its halo, d(v) = 1 + min d(w) over residual arcs v -> w.
+// Halo labels only decrease from launch to launch, so the recomputed labels
+// never exceed the previous ones; a changed border label flags the neighbours.
+__global__ __launch_bounds__(WT) void k_bfs_wave(BP p, WP w, const unsigned long long* __restrict__ planes,
+                                                 int* __restrict__ hgl) {
+  __shared__ int s_lab[WT * (WT + 1)];
+  const int tile = tile_of_block(p.nblocks);
+  if (tile >= p.ntiles) return;
+  const int lane = threadIdx.x;
+  int fl = 0;
+  if (lane == 0) {
+    fl = p.flag_in[tile];
+    p.flag_in[tile] = 0;
+  }
+  if (!__shfl(fl, 0, 64)) return;
+  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
+  const int x0 = tx * WT, y0 = ty * WT, H = p.H, W = p.W;
+  const long long lay = (long long)tz * H;
+  const int y = y0 + lane, xc = x0 + lane;
+  const bool rowok = y < H, colok = xc < W;
+  const bool up = y0 > 0, dn = y0 + WT < H, lf = x0 > 0, rt = x0 + WT < W;
+  unsigned long long P[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) P[s] = rowok ? planes[widx(w, s, tz, ty, tx, lane)] : 0ull;
+  const unsigned long long sink = rowok ? planes[widx(w, 8, tz, ty, tx, lane)] : 0ull;
+  const long long ry = (lay + y) * W;
+  const int hl = (rowok && lf) ? hgl[ry + x0 - 1] : INF_H;
+  const int hr = (rowok && rt) ? hgl[ry + x0 + WT] : INF_H;
+  const long long rtop = (lay + y0 - 1) * W, rbot = (lay + y0 + WT) * W;
+  const int ht = (colok && up) ? hgl[rtop + xc] : INF_H;
+  const int hb = (colok && dn) ? hgl[rbot + xc] : INF_H;
+  const int ctl = (up && lf) ? hgl[rtop + x0 - 1] : INF_H;
+  const int ctr = (up && rt) ? hgl[rtop + x0 + WT] : INF_H;
+  const int cbl = (dn && lf) ? hgl[rbot + x0 - 1] : INF_H;
+  const int cbr = (dn && rt) ? hgl[rbot + x0 + WT] : INF_H;
+  for (int r = 0; r < WT; ++r) {
+    const int yy = y0 + r;
+    s_lab[r * (WT + 1) + lane] = (yy < H && colok) ? hgl[(lay + yy) * W + xc] : INF_H;
+  }
+  __syncthreads();
+
+  // lowest finite halo label >= a (uniform over the wave), INF_H if none
+  auto next_level = [&](int a) {
+    int c = INF_H;
+    c = (hl >= a && hl < c) ? hl : c;
+    c = (hr >= a && hr < c) ? hr : c;
+    c = (ht >= a && ht < c) ? ht : c;
+    c = (hb >= a && hb < c) ? hb : c;
+    c = (ctl >= a && ctl < c) ? ctl : c;
+    c = (ctr >= a && ctr < c) ? ctr : c;
+    c = (cbl >= a && cbl < c) ? cbl : c;
+    c = (cbr >= a && cbr < c) ? cbr : c;
+    return wave_min_int(c);
+  };
+  unsigned long long vis = sink, F = sink, chg = 0;
+  int L = __ballot(sink != 0ull) ? 0 : next_level(0);
+  while (L < INF_H) {
+    // frontier of level L: tile nodes F, halo nodes labelled L
+    const unsigned long long topw = __ballot(ht == L), botw = __ballot(hb == L);
+    const int e = (hl == L ? 1 : 0) | (hr == L ? 2 : 0);
+    unsigned long long Fu = __shfl_up(F, 1, 64), Fd = __shfl_down(F, 1, 64);
+    int eu = __shfl_up(e, 1, 64), ed = __shfl_down(e, 1, 64);
+    if (lane == 0) {
+      Fu = topw;
+      eu = (ctl == L ? 1 : 0) | (ctr == L ? 2 : 0);
+    }
+    if (lane == WT - 1) {
+      Fd = botw;
+      ed = (cbl == L ? 1 : 0) | (cbr == L ? 2 : 0);
+    }
+    // node (y, x) joins level L+1 when its arc towards (y+dy, x+dx) is
+    // residual and that node is in the frontier
+    unsigned long long c = 0;
+    c |= P[0] & ((Fu << 1) | (unsigned long long)(eu & 1));
+    c |= P[1] & Fu;
+    c |= P[2] & ((Fu >> 1) | ((unsigned long long)(eu >> 1) << 63));
+    c |= P[3] & ((F << 1) | (unsigned long long)(e & 1));
+    c |= P[4] & ((F >> 1) | ((unsigned long long)(e >> 1) << 63));
+    c |= P[5] & ((Fd << 1) | (unsigned long long)(ed & 1));
+    c |= P[6] & Fd;
+    c |= P[7] & ((Fd >> 1) | ((unsigned long long)(ed >> 1) << 63));
+    const unsigned long long nw = c & ~vis;
+    vis |= nw;
+    ++L;
+    for (unsigned long long b = nw; b; b &= b - 1) {
+      const int x = __builtin_ctzll(b);
+      int* q = &s_lab[lane * (WT + 1) + x];
+      if (*q != L) {
+        *q = L;
+        chg |= 1ull << x;
+      }
+    }
+    F = nw;
+    if (!__any(nw != 0ull)) L = next_level(L);
+  }
+  __syncthreads();
+  const unsigned long long bord = (lane == 0 || lane == WT - 1) ? chg : (chg & 0x8000000000000001ull);
+  const bool border = __any(bord != 0ull);
+  for (int r = 0; r < WT; ++r) {
+    const unsigned long long cr = __shfl(chg, r, 64);
+    if ((cr >> lane) & 1ull) hgl[(lay + y0 + r) * W + xc] = s_lab[r * (WT + 1) + lane];
+  }
+  if (border && lane == 0) flag_around2d(p, tile);
+}
+
+// One wave tile of the S_min flood (same fixpoint as k_flood_tile): the
+// tile's reached set grows by whole-row closures along the horizontal arcs
+// (one add each way) and one-row steps along the other arcs, fed by the
+// reached halo nodes' arcs into the tile, until it stops changing.
+__global__ __launch_bounds__(WT) void k_flood_wave(BP p, WP w, const unsigned long long* __restrict__ planes,
+                                                   int* __restrict__ reach) {
+  const int tile = tile_of_block(p.nblocks);
+  if (tile >= p.ntiles) return;
+  const int lane = threadIdx.x;
+  int fl = 0;
+  if (lane == 0) {
+    fl = p.flag_in[tile];
+    p.flag_in[tile] = 0;
+  }
+  if (!__shfl(fl, 0, 64)) return;
+  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
+  const int x0 = tx * WT, y0 = ty * WT, H = p.H, W = p.W;
+  const long long lay = (long long)tz * H;
+  const int y = y0 + lane, xc = x0 + lane;
+  const bool rowok = y < H, colok = xc < W;
+  const bool up = y0 > 0, dn = y0 + WT < H, lf = x0 > 0, rt = x0 + WT < W;
+  unsigned long long P[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) P[s] = rowok ? planes[widx(w, s, tz, ty, tx, lane)] : 0ull;
+  // reached nodes of the tile, one ballot per row
+  unsigned long long R = 0;
+  for (int r = 0; r < WT; ++r) {
+    const int yy = y0 + r;
+    const unsigned long long b = __ballot(yy < H && colok && reach[(lay + yy) * W + xc] != 0);
+    if (lane == r) R = b;
+  }
+  const unsigned long long R0 = R;
+
+  // inflow from reached halo nodes (fixed during the launch)
+  const long long ry = (lay + y) * W;
+  // left column node (y, x0-1): arcs (dy, +1) enter (y+dy, 0); bit 63 of the left tile's words
+  int cl = 0, cr = 0;
+  if (rowok && lf && reach[ry + x0 - 1]) {
+    cl = (int)(planes[widx(w, slot_plane(-1, 1), tz, ty, tx - 1, lane)] >> 63) |
+         (int)(planes[widx(w, slot_plane(0, 1), tz, ty, tx - 1, lane)] >> 63) << 1 |
+         (int)(planes[widx(w, slot_plane(1, 1), tz, ty, tx - 1, lane)] >> 63) << 2;
+  }
+  // right column node (y, x0+64): arcs (dy, -1) enter (y+dy, 63); bit 0 of the right tile's words
+  if (rowok && rt && reach[ry + x0 + WT]) {
+    cr = (int)(planes[widx(w, slot_plane(-1, -1), tz, ty, tx + 1, lane)] & 1ull) |
+         (int)(planes[widx(w, slot_plane(0, -1), tz, ty, tx + 1, lane)] & 1ull) << 1 |
+         (int)(planes[widx(w, slot_plane(1, -1), tz, ty, tx + 1, lane)] & 1ull) << 2;
+  }
+  const int clb = __shfl_down(cl, 1, 64), cla = __shfl_up(cl, 1, 64);
+  const int crb = __shfl_down(cr, 1, 64), cra = __shfl_up(cr, 1, 64);
+  unsigned long long hin = 0;
+  // (y+1, x0-1) --(-1,+1)--> (y, 0);  (y, x0-1) --(0,+1)--> (y, 0);  (y-1, x0-1) --(+1,+1)--> (y, 0)
+  if (((lane < WT - 1 ? clb : 0) & 1) | ((cl >> 1) & 1) | ((lane > 0 ? cla : 0) >> 2 & 1)) hin |= 1ull;
+  if (((lane < WT - 1 ? crb : 0) & 1) | ((cr >> 1) & 1) | ((lane > 0 ? cra : 0) >> 2 & 1)) hin |= 1ull << 63;
+  // top halo row y0-1 (last row of the tile above): arcs (+1, dx) enter row 0
+  const unsigned long long rtw = __ballot(up && colok && reach[(lay + y0 - 1) * W + xc] != 0);
+  const unsigned long long rbw = __ballot(dn && colok && reach[(lay + y0 + WT) * W + xc] != 0);
+  if (lane == 0 && up) {
+    const unsigned long long a5 = planes[widx(w, slot_plane(1, -1), tz, ty - 1, tx, WT - 1)];
+    const unsigned long long a6 = planes[widx(w, slot_plane(1, 0), tz, ty - 1, tx, WT - 1)];
+    const unsigned long long a7 = planes[widx(w, slot_plane(1, 1), tz, ty - 1, tx, WT - 1)];
+    hin |= ((rtw & a5) >> 1) | (rtw & a6) | ((rtw & a7) << 1);
+    const long long rtop = (lay + y0 - 1) * W;
+    if (lf && reach[rtop + x0 - 1] && (planes[widx(w, slot_plane(1, 1), tz, ty - 1, tx - 1, WT - 1)] >> 63))
+      hin |= 1ull;
+    if (rt && reach[rtop + x0 + WT] && (planes[widx(w, slot_plane(1, -1), tz, ty - 1, tx + 1, WT - 1)] & 1ull))
+      hin |= 1ull << 63;
+  }
+  // bottom halo row y0+64 (first row of the tile below): arcs (-1, dx) enter row 63
+  if (lane == WT - 1 && dn) {
+    const unsigned long long a0 = planes[widx(w, slot_plane(-1, -1), tz, ty + 1, tx, 0)];
+    const unsigned long long a1 = planes[widx(w, slot_plane(-1, 0), tz, ty + 1, tx, 0)];
+    const unsigned long long a2 = planes[widx(w, slot_plane(-1, 1), tz, ty + 1, tx, 0)];
+    hin |= ((rbw & a0) >> 1) | (rbw & a1) | ((rbw & a2) << 1);
+    const long long rbot = (lay + y0 + WT) * W;
+    if (lf && reach[rbot + x0 - 1] && (planes[widx(w, slot_plane(-1, 1), tz, ty + 1, tx - 1, 0)] >> 63))
+      hin |= 1ull;
+    if (rt && reach[rbot + x0 + WT] && (planes[widx(w, slot_plane(-1, -1), tz, ty + 1, tx + 1, 0)] & 1ull))
+      hin |= 1ull << 63;
+  }
+
+  R |= hin;
+  const unsigned long long gr = P[slot_plane(0, 1)];                         // arcs x -> x+1
+  const unsigned long long gl = __builtin_bitreverse64(P[slot_plane(0, -1)]);  // arcs x -> x-1, mirrored
+  for (;;) {
+    // closure along the row: a run of open arcs carries every seed in it to
+    // the run's far end in one add
+    R |= (gr + (R & gr)) ^ gr;
+    unsigned long long m = __builtin_bitreverse64(R);
+    m |= (gl + (m & gl)) ^ gl;
+    R = __builtin_bitreverse64(m);
+    // one step along the arcs between rows
+    const unsigned long long to_dn = ((R & P[slot_plane(1, -1)]) >> 1) | (R & P[slot_plane(1, 0)]) |
+                                     ((R & P[slot_plane(1, 1)]) << 1);
+    const unsigned long long to_up = ((R & P[slot_plane(-1, -1)]) >> 1) | (R & P[slot_plane(-1, 0)]) |
+                                     ((R & P[slot_plane(-1, 1)]) << 1);
+    unsigned long long from_up = __shfl_up(to_dn, 1, 64), from_dn = __shfl_down(to_up, 1, 64);
+    if (lane == 0) from_up = 0;
+    if (lane == WT - 1) from_dn = 0;
+    const unsigned long long Rn = R | from_up | from_dn;
+    if (!__any(Rn != R)) break;
+    R = Rn;
+  }
+  const unsigned long long add = R & ~R0;
+  const unsigned long long bord = (lane == 0 || lane == WT - 1) ? add : (add & 0x8000000000000001ull);
+  const bool border = __any(bord != 0ull);
+  for (int r = 0; r < WT; ++r) {
+    const unsigned long long a = __shfl(add, r, 64);
+    if ((a >> lane) & 1ull) reach[(lay + y0 + r) * W + xc] = 1;
+  }
+  if (border && lane == 0) flag_around2d(p, tile);
+}
+
 // ---- outputs -------------------------------------------------------------------
 
 // flow = sum(max(red0,0)) - sum(max(-ex,0)), exact in int64
@@ -1755,6 +2051,12 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipMalloc(&bflags_[1], sizeof(int) * bntiles_));
     LF_HIP(hipMalloc(&dev_small_, 64));
     LF_HIP(hipMalloc(&part_, sizeof(long long) * 2 * RED_BLOCKS));
+    if (bits_) {
+      wp_.D = p_.D; wp_.H = p_.H; wp_.W = p_.W; wp_.ntx = bp_.ntx; wp_.nty = bp_.nty;
+      wp_.ps = (long long)p_.D * bp_.nty * bp_.ntx * WT;
+      LF_HIP(hipMalloc(&planes_, sizeof(unsigned long long) * wp_.ps * WPLANES));
+      LF_HIP(hipMemsetAsync(planes_, 0, sizeof(unsigned long long) * wp_.ps * WPLANES, s_));
+    }
     LF_HIP(hipHostMalloc(&host_small_, 64, hipHostMallocDefault));
     LF_HIP(hipEventCreate(&ev_[0]));
     LF_HIP(hipEventCreate(&ev_[1]));
@@ -1764,6 +2066,7 @@ class Engine final : public MinCutEngine {
     hipStreamSynchronize(s_);
     hipFree(red0_); hipFree(alpha0_); hipFree(res_); hipFree(mbox_); hipFree(ex_); hipFree(h_);
     hipFree(mask_); hipFree(reach_); hipFree(flags_[0]); hipFree(flags_[1]); hipFree(bflags_[0]); hipFree(bflags_[1]); hipFree(dev_small_); hipFree(part_);
+    hipFree(planes_);
     hipHostFree(host_small_);
     hipFree(pf_r_); hipFree(pf_reg_); hipFree(pf_tab_); hipFree(pf_nreg_); hipFree(pf_cnt_); hipFree(pf_hist_);
     hipFree(b_rects_); hipFree(b_blk_); hipFree(b_flows_);
@@ -1774,6 +2077,15 @@ class Engine final : public MinCutEngine {
   }
 
   hipStream_t stream() const override { return s_; }
+
+  // Push tiling of a solve: the tuning's tile shape, else the shorter tiles
+  // (G1) for 2D int64 networks, whose residual planes take twice the LDS (the
+  // 32-lambda 1024^2 TV path measured 13.4 s there against 21.5 s on G0),
+  // else G0.
+  int variant_for(const lf_tuning& tun) const {
+    if (tun.tile_shape >= 1 && tun.tile_shape <= 3) return tun.tile_shape - 1;
+    return (sizeof(T) == 8 && B::TD == 1) ? 1 : 0;
+  }
 
   void set_grid(int v) {
     variant_ = v;
@@ -1847,8 +2159,13 @@ class Engine final : public MinCutEngine {
 
   // Maximum preflow of the current working state (res, ex), then S_min.
   // Accumulates into stats.
-  void solve_state(const lf_tuning& tun) {
+  void solve_state(const lf_tuning& tun_in) {
     using clk = std::chrono::steady_clock;
+    // LF_ROUNDS / LF_ITERS / LF_TILE_SHAPE fill tuning fields left at 0 (for studies)
+    lf_tuning tun = tun_in;
+    if (tun.rounds_per_relabel <= 0) tun.rounds_per_relabel = (int)env_double("LF_ROUNDS", 0);
+    if (tun.local_iters <= 0) tun.local_iters = (int)env_double("LF_ITERS", 0);
+    if (tun.tile_shape <= 0) tun.tile_shape = (int)env_double("LF_TILE_SHAPE", 0);
     KP<T> p = params();
     // Fixed tuning when given; otherwise the number of push rounds between two
     // exact relabels follows the measured cost ratio of a relabel and a round
@@ -1858,14 +2175,13 @@ class Engine final : public MinCutEngine {
     const bool fixed_r = tun.rounds_per_relabel > 0;
     int R = fixed_r ? tun.rounds_per_relabel : 4;
     // a solve never switches tilings midway: the mailbox is empty here
-    if (tun.tile_shape >= 1 && tun.tile_shape <= 3) set_grid(tun.tile_shape - 1);
-    else set_grid(0);
+    set_grid(variant_for(tun));
     p = params();
     bool pushed = false;
     long long cycles = 0;
     // Intermediate relabels stop after kRelabelCap launches; the stopping
     // decision is always taken on an exact (uncapped) relabel.
-    constexpr long long kRelabelCap = 16;
+    const long long kRelabelCap = (long long)env_double("LF_RELABEL_CAP", 16);
     bool need_exact = false;
     for (;;) {
       const auto tb = clk::now();
@@ -1931,7 +2247,7 @@ class Engine final : public MinCutEngine {
     const auto tf = clk::now();
     hipLaunchKernelGGL((k_flood_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, reach_);
     run_tiled([&](const BP& q) {
-      hipLaunchKernelGGL((k_flood_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, reach_);
+      launch_flood(q);
       ++stats.flood_launches;
     });
     stats.flood_seconds += std::chrono::duration<double>(clk::now() - tf).count();
@@ -2211,7 +2527,7 @@ class Engine final : public MinCutEngine {
     setup_excess(pf, p, 0, nreg);
     ++levels;
     sections += active;
-    set_grid(tun.tile_shape >= 1 && tun.tile_shape <= 3 ? tun.tile_shape - 1 : 0);
+    set_grid(variant_for(tun));
     p = params();
     const bool fixed_r = tun.rounds_per_relabel > 0;
     int R = fixed_r ? tun.rounds_per_relabel : 4;
@@ -2236,7 +2552,7 @@ class Engine final : public MinCutEngine {
         hipLaunchKernelGGL((k_pf_after<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, 4);
         hipLaunchKernelGGL((k_pf_flood_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, reach_, 4);
         run_tiled([&](const BP& q) {
-          hipLaunchKernelGGL((k_flood_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, reach_);
+          launch_flood(q);
           ++stats.flood_launches;
         });
         hipLaunchKernelGGL((k_pf_stats_impl<1>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)reach_, 4);
@@ -2544,7 +2860,17 @@ class Engine final : public MinCutEngine {
       }
       LF_HIP(hipGetLastError());
       if (read_counter() == 0) return true;
+      // labels / reach bits only move one way, so the fixpoint is reached;
+      // a pass that does not get there within this many launches is a bug
+      if (launched > 16 * bntiles_ + 4096) fail(LF_ERR_STATE, "tiled exact pass did not converge");
     }
+  }
+
+  void launch_flood(const BP& q) {
+    if (bits_)
+      hipLaunchKernelGGL(k_flood_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, reach_);
+    else
+      hipLaunchKernelGGL((k_flood_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, reach_);
   }
 
   // Deliver mailboxes, BFS labels, flag the tiles of active nodes; returns the
@@ -2558,11 +2884,21 @@ class Engine final : public MinCutEngine {
         using GG = decltype(g);
         hipLaunchKernelGGL((k_deliver<K, GG, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
       });
-    hipLaunchKernelGGL((k_bfs_init<K, T, MT>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, mask_);
-    *exact = run_tiled([&](const BP& q) {
-      hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, h_);
-      ++stats.bfs_launches;
-    }, cap);
+    if (bits_) {
+      const long long nw = (long long)p_.D * p_.H * wp_.ntx;  // one wave per plane word
+      hipLaunchKernelGGL((k_bits_init<K, T>), dim3((unsigned)((nw * 64 + NTHREADS - 1) / NTHREADS)), dim3(NTHREADS),
+                         0, s_, p, wp_, planes_);
+      *exact = run_tiled([&](const BP& q) {
+        hipLaunchKernelGGL(k_bfs_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, h_);
+        ++stats.bfs_launches;
+      }, cap);
+    } else {
+      hipLaunchKernelGGL((k_bfs_init<K, T, MT>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, mask_);
+      *exact = run_tiled([&](const BP& q) {
+        hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, h_);
+        ++stats.bfs_launches;
+      }, cap);
+    }
     // push rounds start with exactly the tiles holding active nodes
     fill_flags(flags_[cur_], 0);
     fill_flags(flags_[cur_ ^ 1], 0);
@@ -2588,6 +2924,12 @@ class Engine final : public MinCutEngine {
   T* ex_ = nullptr;
   int* h_ = nullptr;
   MT* mask_ = nullptr;
+  // LF_WAVE_BFS=1: 2D lattices with 3x3 neighbourhoods run the exact passes
+  // on wave tiles over arc bit planes instead of the LDS-list tile kernels
+  const bool bits_ = B::TD == 1 && B::TH == WT && B::TW == WT && B::RZ == 0 && B::RY == 1 && B::RX == 1 &&
+                     std::getenv("LF_WAVE_BFS") != nullptr;
+  WP wp_{};
+  unsigned long long* planes_ = nullptr;
   int* reach_ = nullptr;
   int* flags_[2] = {nullptr, nullptr};
   int cur_ = 0;
