@@ -77,6 +77,14 @@ struct KP {
   int* flag_out;
   unsigned long long* counter;
   unsigned long long* tiles;  // processed-tile counter of the push rounds
+  // push work lists: the round takes its tiles from wl_in (wl_cnt_in of
+  // them) and appends the tiles of the next round to wl_out; wl_cnt_zero is
+  // the counter the round after next appends to, cleared here
+  const int* wl_in;
+  int* wl_out;
+  const unsigned* wl_cnt_in;
+  unsigned* wl_cnt_out;
+  unsigned* wl_cnt_zero;
   int D, H, W;
   long long N;
   int ntx, nty, ntz, ntiles, nblocks;
@@ -207,6 +215,14 @@ __device__ inline void tile_coords(int t, int& z, int& y, int& x) {
   x = t % G::TW;
 }
 
+// Append tile t to a push work list once per round (its flag dedups).
+__device__ inline void wl_append(int* flags, int* list, unsigned* cnt, int t) {
+  if (atomicExch(&flags[t], 1) == 0) list[atomicAdd(cnt, 1u)] = t;
+}
+
+// One push / relabel round over the tiles of the work list; a fixed grid of
+// resident blocks walks the list, so a round over few active tiles costs
+// little more than those tiles.
 template <int K, class G, typename T>
 __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   constexpr int NT = G::NT;
@@ -216,199 +232,224 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   __shared__ unsigned short s_list[NT];
   __shared__ unsigned int s_dirty[NT];  // bit i: residual plane i changed; bit 31: excess or label
   __shared__ int s_cnt[2];
-  __shared__ int s_flag;
+  __shared__ int s_tile;
+  __shared__ unsigned int s_nbr;  // bit 9 (sz+1) + 3 (sy+1) + (sx+1): neighbouring tile received flow
   static_assert(K <= 31, "dirty mask holds 31 planes");
 
-  const int tile = tile_of_block(p.nblocks);
-  if (tile >= p.ntiles) return;
-  if (threadIdx.x == 0) {
-    s_flag = p.flag_in[tile];
-    p.flag_in[tile] = 0;
-    s_cnt[0] = 0;
-    s_cnt[1] = 0;
-  }
-  __syncthreads();
-  if (!s_flag) return;
-  if (threadIdx.x == 0) atomicAdd(p.tiles, 1ull);
-
-  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
-  const int x0 = tx * G::TW, y0 = ty * G::TH, z0 = tz * G::TD;
-  const long long N = p.N;
-  // lattice extent of this tile (nodes past the lattice edge are never active)
-  const int lw = min(G::TW, p.W - x0), lh = min(G::TH, p.H - y0), ld = min(G::TD, p.D - z0);
-
-  // stage residuals and excess of the tile, labels with a halo
-#pragma unroll
-  for (int j = 0; j < G::NPT; ++j) {
-    const int t = threadIdx.x + j * NTHREADS;
-    int z, y, x;
-    tile_coords<G>(t, z, y, x);
-    if (z < ld && y < lh && x < lw) {
-      const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
-#pragma unroll
-      for (int i = 0; i < K; ++i) s_res[i * NT + t] = p.res[i * N + g];
-      s_ex[t] = p.ex[g];
-    } else {
-#pragma unroll
-      for (int i = 0; i < K; ++i) s_res[i * NT + t] = 0;
-      s_ex[t] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p.wl_cnt_zero = 0;
+  const int ntodo = (int)*p.wl_cnt_in;
+  for (int li = blockIdx.x; li < ntodo; li += gridDim.x) {
+    if (threadIdx.x == 0) {
+      const int t = p.wl_in[li];
+      s_tile = t;
+      p.flag_in[t] = 0;
+      s_cnt[0] = 0;
+      s_cnt[1] = 0;
+      s_nbr = 0;
+      atomicAdd(p.tiles, 1ull);
     }
-    s_dirty[t] = 0;
-  }
-  for (int q = threadIdx.x; q < G::NH; q += NTHREADS) {
-    const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
-    const int gz = z0 + hz - G::RZ, gy = y0 + hy - G::RY, gx = x0 + hx - G::RX;
-    int v = INF_H;
-    if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W)
-      v = p.h[((long long)gz * p.H + gy) * p.W + gx];
-    s_h[q] = v;
-  }
+    __syncthreads();
+    const int tile = s_tile;
 
-  // collect flow that neighbouring tiles pushed into this tile
-#pragma unroll
-  for (int j = 0; j < G::NPT; ++j) {
-    const int t = threadIdx.x + j * NTHREADS;
-    int z, y, x;
-    tile_coords<G>(t, z, y, x);
-    const bool border = z < G::RZ || z >= G::TD - G::RZ || y < G::RY || y >= G::TH - G::RY ||
-                        x < G::RX || x >= G::TW - G::RX;
-    if (!border || z >= ld || y >= lh || x >= lw) continue;
-    const int gz = z0 + z, gy = y0 + y, gx = x0 + x;
-    const long long g = ((long long)gz * p.H + gy) * p.W + gx;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
-      const int lz = z + dz, ly = y + dy, lx = x + dx;
-      if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) continue;
-      const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
-      if (nz < 0 || nz >= p.D || ny < 0 || ny >= p.H || nx < 0 || nx >= p.W) continue;
-      T* slot = &p.mbox[(long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx];
-      if (__builtin_nontemporal_load(slot) != 0) {
-        const T a = gtake(slot);
-        s_ex[t] += a;
-        s_res[i * NT + t] += a;
-        s_dirty[t] |= (1u << i) | 0x80000000u;
-      }
-    }
-  }
-  __syncthreads();
+    const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
+    const int x0 = tx * G::TW, y0 = ty * G::TH, z0 = tz * G::TD;
+    const long long N = p.N;
+    // lattice extent of this tile (nodes past the lattice edge are never active)
+    const int lw = min(G::TW, p.W - x0), lh = min(G::TH, p.H - y0), ld = min(G::TD, p.D - z0);
 
-  for (int it = 0; it < p.iters; ++it) {
-    // list the active nodes (ex > 0, label < INF) with one LDS atomic per wave
-    int* cnt = &s_cnt[it & 1];
+    // stage residuals and excess of the tile, labels with a halo
 #pragma unroll
     for (int j = 0; j < G::NPT; ++j) {
       const int t = threadIdx.x + j * NTHREADS;
-      const T e = s_ex[t];
       int z, y, x;
       tile_coords<G>(t, z, y, x);
-      const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
-      const bool act = e > 0 && s_h[hi] < INF_H;
-      const unsigned long long b = __ballot(act);
-      const int lane = threadIdx.x & 63;
-      int base = 0;
-      if (lane == 0 && b) base = atomicAdd(cnt, __popcll(b));
-      base = __shfl(base, 0, 64);
-      if (act) s_list[base + __popcll(b & ((1ull << lane) - 1))] = (unsigned short)t;
+      if (z < ld && y < lh && x < lw) {
+        const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+#pragma unroll
+        for (int i = 0; i < K; ++i) s_res[i * NT + t] = p.res[i * N + g];
+        s_ex[t] = p.ex[g];
+      } else {
+#pragma unroll
+        for (int i = 0; i < K; ++i) s_res[i * NT + t] = 0;
+        s_ex[t] = 0;
+      }
+      s_dirty[t] = 0;
     }
-    __syncthreads();
-    const int n = *cnt;
-    if (n == 0) break;
-    if (threadIdx.x == 0) s_cnt[(it + 1) & 1] = 0;
+    for (int q = threadIdx.x; q < G::NH; q += NTHREADS) {
+      const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
+      const int gz = z0 + hz - G::RZ, gy = y0 + hy - G::RY, gx = x0 + hx - G::RX;
+      int v = INF_H;
+      if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W)
+        v = p.h[((long long)gz * p.H + gy) * p.W + gx];
+      s_h[q] = v;
+    }
 
-    // push along downhill residual arcs, labels frozen
-    for (int q = threadIdx.x; q < n; q += NTHREADS) {
-      const int t = s_list[q];
+    // collect flow that neighbouring tiles pushed into this tile
+#pragma unroll
+    for (int j = 0; j < G::NPT; ++j) {
+      const int t = threadIdx.x + j * NTHREADS;
       int z, y, x;
       tile_coords<G>(t, z, y, x);
-      const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
-      const int hv = s_h[hi];
-      // excess moves only through LDS atomics in this phase: in-tile receivers
-      // add, the owner subtracts what it pushed
-      const T e0 = s_ex[t];
-      T e = e0;
-      unsigned int dirty = 0;
+      const bool border = z < G::RZ || z >= G::TD - G::RZ || y < G::RY || y >= G::TH - G::RY ||
+                          x < G::RX || x >= G::TW - G::RX;
+      if (!border || z >= ld || y >= lh || x >= lw) continue;
+      const int gz = z0 + z, gy = y0 + y, gx = x0 + x;
+      const long long g = ((long long)gz * p.H + gy) * p.W + gx;
 #pragma unroll
       for (int i = 0; i < K; ++i) {
         const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
-        const int hn = s_h[hi + (dz * G::HH + dy) * G::HW + dx];
-        if (e > 0 && hn < hv) {
-          const T r = s_res[i * NT + t];
-          if (r > 0) {
-            const T d = r < e ? r : e;
-            s_res[i * NT + t] = r - d;
-            e -= d;
-            dirty |= 1u << i;
-            const int lz = z + dz, ly = y + dy, lx = x + dx;
-            if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) {
-              const int tn = t + (dz * G::TH + dy) * G::TW + dx;
-              s_res[((i + K / 2) % K) * NT + tn] += d;
-              lds_add(&s_ex[tn], d);
-              atomicOr(&s_dirty[tn], (1u << ((i + K / 2) % K)) | 0x80000000u);
-            } else {
-              const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
-              gadd(&p.mbox[(long long)i * N + g], d);
-              flag_neighbor_tile<G, T>(p, z0 + lz, y0 + ly, x0 + lx, p.flag_out);
+        const int lz = z + dz, ly = y + dy, lx = x + dx;
+        if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) continue;
+        const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
+        if (nz < 0 || nz >= p.D || ny < 0 || ny >= p.H || nx < 0 || nx >= p.W) continue;
+        T* slot = &p.mbox[(long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx];
+        if (__builtin_nontemporal_load(slot) != 0) {
+          const T a = gtake(slot);
+          s_ex[t] += a;
+          s_res[i * NT + t] += a;
+          s_dirty[t] |= (1u << i) | 0x80000000u;
+        }
+      }
+    }
+    __syncthreads();
+
+    for (int it = 0; it < p.iters; ++it) {
+      // list the active nodes (ex > 0, label < INF) with one LDS atomic per wave
+      int* cnt = &s_cnt[it & 1];
+#pragma unroll
+      for (int j = 0; j < G::NPT; ++j) {
+        const int t = threadIdx.x + j * NTHREADS;
+        const T e = s_ex[t];
+        int z, y, x;
+        tile_coords<G>(t, z, y, x);
+        const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
+        const bool act = e > 0 && s_h[hi] < INF_H;
+        const unsigned long long b = __ballot(act);
+        const int lane = threadIdx.x & 63;
+        int base = 0;
+        if (lane == 0 && b) base = atomicAdd(cnt, __popcll(b));
+        base = __shfl(base, 0, 64);
+        if (act) s_list[base + __popcll(b & ((1ull << lane) - 1))] = (unsigned short)t;
+      }
+      __syncthreads();
+      const int n = *cnt;
+      if (n == 0) break;
+      if (threadIdx.x == 0) s_cnt[(it + 1) & 1] = 0;
+
+      // push along downhill residual arcs, labels frozen
+      for (int q = threadIdx.x; q < n; q += NTHREADS) {
+        const int t = s_list[q];
+        int z, y, x;
+        tile_coords<G>(t, z, y, x);
+        const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
+        const int hv = s_h[hi];
+        // excess moves only through LDS atomics in this phase: in-tile receivers
+        // add, the owner subtracts what it pushed
+        const T e0 = s_ex[t];
+        T e = e0;
+        unsigned int dirty = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+          const int hn = s_h[hi + (dz * G::HH + dy) * G::HW + dx];
+          if (e > 0 && hn < hv) {
+            const T r = s_res[i * NT + t];
+            if (r > 0) {
+              const T d = r < e ? r : e;
+              s_res[i * NT + t] = r - d;
+              e -= d;
+              dirty |= 1u << i;
+              const int lz = z + dz, ly = y + dy, lx = x + dx;
+              if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) {
+                const int tn = t + (dz * G::TH + dy) * G::TW + dx;
+                s_res[((i + K / 2) % K) * NT + tn] += d;
+                lds_add(&s_ex[tn], d);
+                atomicOr(&s_dirty[tn], (1u << ((i + K / 2) % K)) | 0x80000000u);
+              } else {
+                const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+                gadd(&p.mbox[(long long)i * N + g], d);
+                const int sz = lz < 0 ? 0 : (lz >= G::TD ? 2 : 1), sy = ly < 0 ? 0 : (ly >= G::TH ? 2 : 1),
+                          sx = lx < 0 ? 0 : (lx >= G::TW ? 2 : 1);
+                atomicOr(&s_nbr, 1u << (sz * 9 + sy * 3 + sx));
+              }
             }
           }
         }
-      }
-      if (e != e0) {
-        lds_add(&s_ex[t], e - e0);
-        atomicOr(&s_dirty[t], dirty | 0x80000000u);
-      }
-    }
-    __syncthreads();
-
-    // relabel listed nodes that kept excess, residuals frozen; labels only
-    // grow, so concurrent reads of a neighbour's label are safe
-    for (int q = threadIdx.x; q < n; q += NTHREADS) {
-      const int t = s_list[q];
-      if (s_ex[t] <= 0) continue;
-      int z, y, x;
-      tile_coords<G>(t, z, y, x);
-      const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
-      const int hv = s_h[hi];
-      int m = INF_H;
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        if (s_res[i * NT + t] > 0) {
-          const int hn = s_h[hi + (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i]];
-          m = hn < m ? hn : m;
+        if (e != e0) {
+          lds_add(&s_ex[t], e - e0);
+          atomicOr(&s_dirty[t], dirty | 0x80000000u);
         }
       }
-      if (m >= hv) {
-        s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
-        atomicOr(&s_dirty[t], 0x80000000u);
+      __syncthreads();
+
+      // relabel listed nodes that kept excess, residuals frozen; labels only
+      // grow, so concurrent reads of a neighbour's label are safe
+      for (int q = threadIdx.x; q < n; q += NTHREADS) {
+        const int t = s_list[q];
+        if (s_ex[t] <= 0) continue;
+        int z, y, x;
+        tile_coords<G>(t, z, y, x);
+        const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
+        const int hv = s_h[hi];
+        int m = INF_H;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          if (s_res[i * NT + t] > 0) {
+            const int hn = s_h[hi + (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i]];
+            m = hn < m ? hn : m;
+          }
+        }
+        if (m >= hv) {
+          s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
+          atomicOr(&s_dirty[t], 0x80000000u);
+        }
+      }
+      __syncthreads();
+    }
+
+    // write back what changed (residual planes, excess and label of touched nodes)
+    bool act = false;
+#pragma unroll
+    for (int j = 0; j < G::NPT; ++j) {
+      const int t = threadIdx.x + j * NTHREADS;
+      int z, y, x;
+      tile_coords<G>(t, z, y, x);
+      if (z >= ld || y >= lh || x >= lw) continue;
+      const unsigned int dm = s_dirty[t];
+      const T e = s_ex[t];
+      const int hv = s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)];
+      if (dm) {
+        const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+          if (dm & (1u << i)) p.res[i * N + g] = s_res[i * NT + t];
+        p.ex[g] = e;
+        p.h[g] = hv;
+      }
+      if (e > 0 && hv < INF_H) act = true;
+    }
+    const bool self_active = __syncthreads_or(act);
+    if (threadIdx.x == 0) {
+      // next round: this tile while it holds active nodes, and every tile
+      // whose mailbox it filled
+      unsigned m = s_nbr | (self_active ? 1u << 13 : 0u);
+      while (m) {
+        const int b = __builtin_ctz(m);
+        m &= m - 1;
+        const int nz = tz + b / 9 - 1, ny = ty + (b / 3) % 3 - 1, nx = tx + b % 3 - 1;
+        if (nz < 0 || nz >= p.ntz || ny < 0 || ny >= p.nty || nx < 0 || nx >= p.ntx) continue;
+        wl_append(p.flag_out, p.wl_out, p.wl_cnt_out, (nz * p.nty + ny) * p.ntx + nx);
       }
     }
     __syncthreads();
   }
+}
 
-  // write back what changed (residual planes, excess and label of touched nodes)
-  bool act = false;
-#pragma unroll
-  for (int j = 0; j < G::NPT; ++j) {
-    const int t = threadIdx.x + j * NTHREADS;
-    int z, y, x;
-    tile_coords<G>(t, z, y, x);
-    if (z >= ld || y >= lh || x >= lw) continue;
-    const unsigned int dm = s_dirty[t];
-    const T e = s_ex[t];
-    const int hv = s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)];
-    if (dm) {
-      const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
-#pragma unroll
-      for (int i = 0; i < K; ++i)
-        if (dm & (1u << i)) p.res[i * N + g] = s_res[i * NT + t];
-      p.ex[g] = e;
-      p.h[g] = hv;
-    }
-    if (e > 0 && hv < INF_H) act = true;
-  }
-  if (__syncthreads_or(act)) {
-    if (threadIdx.x == 0) p.flag_out[tile] = 1;
-  }
+// Work list of the tiles whose flag is set (the first round after a relabel).
+__global__ __launch_bounds__(NTHREADS) void k_flags_to_list(const int* __restrict__ flags, int ntiles, int* list,
+                                                            unsigned* cnt) {
+  const int t = blockIdx.x * NTHREADS + threadIdx.x;
+  if (t < ntiles && flags[t]) list[atomicAdd(cnt, 1u)] = t;
 }
 
 // ---- mailbox delivery (before any exact pass) --------------------------------
@@ -2047,6 +2088,24 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipMalloc(&reach_, sizeof(int) * N_));
     LF_HIP(hipMalloc(&flags_[0], sizeof(int) * ntiles_));
     LF_HIP(hipMalloc(&flags_[1], sizeof(int) * ntiles_));
+    LF_HIP(hipMalloc(&wl_[0], sizeof(int) * ntiles_));
+    LF_HIP(hipMalloc(&wl_[1], sizeof(int) * ntiles_));
+    LF_HIP(hipMalloc(&wl_cnt_, 4 * sizeof(unsigned)));
+    LF_HIP(hipMemsetAsync(wl_cnt_, 0, 4 * sizeof(unsigned), s_));
+    {
+      // push rounds run as many blocks as the device holds at once
+      int dev = 0, cus = 0;
+      LF_HIP(hipGetDevice(&dev));
+      LF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const void* fn[3] = {reinterpret_cast<const void*>(&k_push<K, G0, T>),
+                           reinterpret_cast<const void*>(&k_push<K, G1, T>),
+                           reinterpret_cast<const void*>(&k_push<K, G2, T>)};
+      for (int v = 0; v < 3; ++v) {
+        int nb = 0;
+        LF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn[v], NTHREADS, 0));
+        wl_grid_[v] = std::max(1, nb) * std::max(1, cus);
+      }
+    }
     LF_HIP(hipMalloc(&bflags_[0], sizeof(int) * bntiles_));
     LF_HIP(hipMalloc(&bflags_[1], sizeof(int) * bntiles_));
     LF_HIP(hipMalloc(&dev_small_, 64));
@@ -2066,7 +2125,7 @@ class Engine final : public MinCutEngine {
     hipStreamSynchronize(s_);
     hipFree(red0_); hipFree(alpha0_); hipFree(res_); hipFree(mbox_); hipFree(ex_); hipFree(h_);
     hipFree(mask_); hipFree(reach_); hipFree(flags_[0]); hipFree(flags_[1]); hipFree(bflags_[0]); hipFree(bflags_[1]); hipFree(dev_small_); hipFree(part_);
-    hipFree(planes_);
+    hipFree(planes_); hipFree(wl_[0]); hipFree(wl_[1]); hipFree(wl_cnt_);
     hipHostFree(host_small_);
     hipFree(pf_r_); hipFree(pf_reg_); hipFree(pf_tab_); hipFree(pf_nreg_); hipFree(pf_cnt_); hipFree(pf_hist_);
     hipFree(b_rects_); hipFree(b_blk_); hipFree(b_flows_);
@@ -2216,17 +2275,7 @@ class Engine final : public MinCutEngine {
         const unsigned long long tiles0 = fixed_r ? 0 : read_tiles();
         LF_HIP(hipEventRecord(ev_[0], s_));
         const int n = std::min(chunk, R - done_rounds);
-        for (int r = 0; r < n; ++r) {
-          KP<T> q = p;
-          q.flag_in = flags_[cur_];
-          q.flag_out = flags_[cur_ ^ 1];
-          with_geo([&](auto g) {
-            using GG = decltype(g);
-            hipLaunchKernelGGL((k_push<K, GG, T>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q);
-          });
-          cur_ ^= 1;
-          ++stats.rounds;
-        }
+        for (int r = 0; r < n; ++r) launch_push_round(p);
         LF_HIP(hipGetLastError());
         LF_HIP(hipEventRecord(ev_[1], s_));
         pushed = true;
@@ -2582,17 +2631,7 @@ class Engine final : public MinCutEngine {
         const unsigned long long tiles0 = fixed_r ? 0 : read_tiles();
         LF_HIP(hipEventRecord(ev_[0], s_));
         const int n = std::min(chunk, R - done_rounds);
-        for (int r = 0; r < n; ++r) {
-          KP<T> q = p;
-          q.flag_in = flags_[cur_];
-          q.flag_out = flags_[cur_ ^ 1];
-          with_geo([&](auto g) {
-            using GG = decltype(g);
-            hipLaunchKernelGGL((k_push<K, GG, T>), dim3(p_.nblocks), dim3(NTHREADS), 0, s_, q);
-          });
-          cur_ ^= 1;
-          ++stats.rounds;
-        }
+        for (int r = 0; r < n; ++r) launch_push_round(p);
         LF_HIP(hipGetLastError());
         LF_HIP(hipEventRecord(ev_[1], s_));
         pushed = true;
@@ -2866,6 +2905,26 @@ class Engine final : public MinCutEngine {
     }
   }
 
+  // One push / relabel round over the current work list; the next round's
+  // list is the other one, and the three list counters rotate (in, out, to clear).
+  void launch_push_round(const KP<T>& p) {
+    KP<T> q = p;
+    q.flag_in = flags_[cur_];
+    q.flag_out = flags_[cur_ ^ 1];
+    q.wl_in = wl_[cur_];
+    q.wl_out = wl_[cur_ ^ 1];
+    q.wl_cnt_in = wl_cnt_ + wl_k_ % 3;
+    q.wl_cnt_out = wl_cnt_ + (wl_k_ + 1) % 3;
+    q.wl_cnt_zero = wl_cnt_ + (wl_k_ + 2) % 3;
+    with_geo([&](auto g) {
+      using GG = decltype(g);
+      hipLaunchKernelGGL((k_push<K, GG, T>), dim3(wl_grid_[variant_]), dim3(NTHREADS), 0, s_, q);
+    });
+    cur_ ^= 1;
+    wl_k_ = (wl_k_ + 1) % 3;
+    ++stats.rounds;
+  }
+
   void launch_flood(const BP& q) {
     if (bits_)
       hipLaunchKernelGGL(k_flood_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, reach_);
@@ -2908,6 +2967,10 @@ class Engine final : public MinCutEngine {
       using GG = decltype(g);
       hipLaunchKernelGGL((k_count_active<GG, T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, q, part_);
     });
+    // the first round's work list: the flagged tiles
+    hipLaunchKernelGGL((k_fill<unsigned>), dim3(1), dim3(NTHREADS), 0, s_, wl_cnt_, 0u, 3LL);
+    hipLaunchKernelGGL((k_flags_to_list), dim3(blocks_for(p_.ntiles)), dim3(NTHREADS), 0, s_, flags_[cur_],
+                       p_.ntiles, wl_[cur_], wl_cnt_ + wl_k_);
     hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
     LF_HIP(hipGetLastError());
     return (long long)read_counter();
@@ -2932,6 +2995,10 @@ class Engine final : public MinCutEngine {
   unsigned long long* planes_ = nullptr;
   int* reach_ = nullptr;
   int* flags_[2] = {nullptr, nullptr};
+  int* wl_[2] = {nullptr, nullptr};  // push work lists (tile ids)
+  unsigned* wl_cnt_ = nullptr;       // their lengths, rotating over three counters
+  int wl_k_ = 0;                     // counter index of the next round's input list
+  int wl_grid_[3] = {1, 1, 1};       // resident blocks of k_push per tiling
   int cur_ = 0;
   BP bp_{};
   TileGrid grid_[3];
