@@ -50,6 +50,8 @@ namespace lfamd {
 namespace {
 
 constexpr int NTHREADS = 256;
+// directions whose LDS operands the push round reads in one batch
+constexpr int HOIST = 8;
 constexpr int INF_H = 1 << 30;
 constexpr int MAXK = 32;
 
@@ -257,34 +259,34 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
     // lattice extent of this tile (nodes past the lattice edge are never active)
     const int lw = min(G::TW, p.W - x0), lh = min(G::TH, p.H - y0), ld = min(G::TD, p.D - z0);
 
-    // stage residuals and excess of the tile, labels with a halo
+    // stage residuals and excess of the tile.  Slots past the lattice edge
+    // load node 0 and discard it: no load waits behind a branch, so a
+    // thread's whole batch of loads is in flight at once.
+    {
+      T r[G::NPT][K + 1];
+      bool in[G::NPT];
 #pragma unroll
-    for (int j = 0; j < G::NPT; ++j) {
-      const int t = threadIdx.x + j * NTHREADS;
-      int z, y, x;
-      tile_coords<G>(t, z, y, x);
-      if (z < ld && y < lh && x < lw) {
-        const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+      for (int j = 0; j < G::NPT; ++j) {
+        int z, y, x;
+        tile_coords<G>(threadIdx.x + j * NTHREADS, z, y, x);
+        in[j] = z < ld && y < lh && x < lw;
+        const long long g = in[j] ? ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x) : 0;
 #pragma unroll
-        for (int i = 0; i < K; ++i) s_res[i * NT + t] = p.res[i * N + g];
-        s_ex[t] = p.ex[g];
-      } else {
-#pragma unroll
-        for (int i = 0; i < K; ++i) s_res[i * NT + t] = 0;
-        s_ex[t] = 0;
+        for (int i = 0; i < K; ++i) r[j][i] = p.res[i * N + g];
+        r[j][K] = p.ex[g];
       }
-      s_dirty[t] = 0;
+#pragma unroll
+      for (int j = 0; j < G::NPT; ++j) {
+        const int t = threadIdx.x + j * NTHREADS;
+#pragma unroll
+        for (int i = 0; i < K; ++i) s_res[i * NT + t] = in[j] ? r[j][i] : (T)0;
+        s_ex[t] = in[j] ? r[j][K] : (T)0;
+        s_dirty[t] = 0;
+      }
     }
-    for (int q = threadIdx.x; q < G::NH; q += NTHREADS) {
-      const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
-      const int gz = z0 + hz - G::RZ, gy = y0 + hy - G::RY, gx = x0 + hx - G::RX;
-      int v = INF_H;
-      if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W)
-        v = p.h[((long long)gz * p.H + gy) * p.W + gx];
-      s_h[q] = v;
-    }
-
-    // collect flow that neighbouring tiles pushed into this tile
+    // collect the flow that neighbouring tiles pushed into this tile: plain
+    // loads find the non-empty slots, which are then taken with atomic
+    // exchanges (a sender of this round may still add to them)
 #pragma unroll
     for (int j = 0; j < G::NPT; ++j) {
       const int t = threadIdx.x + j * NTHREADS;
@@ -294,22 +296,49 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
                           x < G::RX || x >= G::TW - G::RX;
       if (!border || z >= ld || y >= lh || x >= lw) continue;
       const int gz = z0 + z, gy = y0 + y, gx = x0 + x;
-      const long long g = ((long long)gz * p.H + gy) * p.W + gx;
+      T add_e = 0;
+      unsigned int dirty = 0;
 #pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
-        const int lz = z + dz, ly = y + dy, lx = x + dx;
-        if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) continue;
-        const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
-        if (nz < 0 || nz >= p.D || ny < 0 || ny >= p.H || nx < 0 || nx >= p.W) continue;
-        T* slot = &p.mbox[(long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx];
-        if (__builtin_nontemporal_load(slot) != 0) {
-          const T a = gtake(slot);
-          s_ex[t] += a;
-          s_res[i * NT + t] += a;
-          s_dirty[t] |= (1u << i) | 0x80000000u;
+      for (int c = 0; c < K; c += HOIST) {
+        long long slot[HOIST];
+        T a[HOIST];
+#pragma unroll
+        for (int u = 0; u < HOIST && c + u < K; ++u) {
+          const int i = c + u;
+          const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+          const int lz = z + dz, ly = y + dy, lx = x + dx;
+          const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
+          const bool from_out = !(lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) &&
+                                nz >= 0 && nz < p.D && ny >= 0 && ny < p.H && nx >= 0 && nx < p.W;
+          slot[u] = from_out ? (long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx : -1;
+          const T hint = __builtin_nontemporal_load(&p.mbox[slot[u] < 0 ? 0 : slot[u]]);
+          a[u] = slot[u] < 0 ? (T)0 : hint;
+        }
+#pragma unroll
+        for (int u = 0; u < HOIST && c + u < K; ++u)
+          if (a[u] != 0) a[u] = gtake(&p.mbox[slot[u]]);
+#pragma unroll
+        for (int u = 0; u < HOIST && c + u < K; ++u) {
+          if (a[u] == 0) continue;
+          const int i = c + u;
+          add_e += a[u];
+          s_res[i * NT + t] += a[u];  // only this thread touches node t's planes while staging
+          dirty |= (1u << i) | 0x80000000u;
         }
       }
+      if (dirty) {
+        s_ex[t] += add_e;
+        s_dirty[t] = dirty;
+      }
+    }
+#pragma unroll
+    for (int q0 = 0; q0 < G::NH; q0 += NTHREADS) {
+      const int q = q0 + threadIdx.x;
+      const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
+      const int gz = z0 + hz - G::RZ, gy = y0 + hy - G::RY, gx = x0 + hx - G::RX;
+      const bool in = q < G::NH && gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W;
+      const int v = p.h[in ? ((long long)gz * p.H + gy) * p.W + gx : 0];
+      if (q < G::NH) s_h[q] = in ? v : INF_H;
     }
     __syncthreads();
 
@@ -346,31 +375,46 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         // excess moves only through LDS atomics in this phase: in-tile receivers
         // add, the owner subtracts what it pushed
         const T e0 = s_ex[t];
+        // the labels and residuals the node looks at, read in batches of HOIST
+        // directions (labels are frozen in this phase; only this thread lowers
+        // node t's residuals, and a receiver's increase lands on an arc that t
+        // does not push along, since both ends cannot be lower than the other)
         T e = e0;
         unsigned int dirty = 0;
 #pragma unroll
-        for (int i = 0; i < K; ++i) {
-          const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
-          const int hn = s_h[hi + (dz * G::HH + dy) * G::HW + dx];
-          if (e > 0 && hn < hv) {
-            const T r = s_res[i * NT + t];
-            if (r > 0) {
-              const T d = r < e ? r : e;
-              s_res[i * NT + t] = r - d;
-              e -= d;
-              dirty |= 1u << i;
-              const int lz = z + dz, ly = y + dy, lx = x + dx;
-              if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) {
-                const int tn = t + (dz * G::TH + dy) * G::TW + dx;
-                s_res[((i + K / 2) % K) * NT + tn] += d;
-                lds_add(&s_ex[tn], d);
-                atomicOr(&s_dirty[tn], (1u << ((i + K / 2) % K)) | 0x80000000u);
-              } else {
-                const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
-                gadd(&p.mbox[(long long)i * N + g], d);
-                const int sz = lz < 0 ? 0 : (lz >= G::TD ? 2 : 1), sy = ly < 0 ? 0 : (ly >= G::TH ? 2 : 1),
-                          sx = lx < 0 ? 0 : (lx >= G::TW ? 2 : 1);
-                atomicOr(&s_nbr, 1u << (sz * 9 + sy * 3 + sx));
+        for (int c = 0; c < K; c += HOIST) {
+          int hn[HOIST];
+          T rr[HOIST];
+#pragma unroll
+          for (int u = 0; u < HOIST && c + u < K; ++u) {
+            const int i = c + u;
+            hn[u] = s_h[hi + (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i]];
+            rr[u] = s_res[i * NT + t];
+          }
+#pragma unroll
+          for (int u = 0; u < HOIST && c + u < K; ++u) {
+            const int i = c + u;
+            const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+            if (e > 0 && hn[u] < hv) {
+              const T r = rr[u];
+              if (r > 0) {
+                const T d = r < e ? r : e;
+                s_res[i * NT + t] = r - d;
+                e -= d;
+                dirty |= 1u << i;
+                const int lz = z + dz, ly = y + dy, lx = x + dx;
+                if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) {
+                  const int tn = t + (dz * G::TH + dy) * G::TW + dx;
+                  lds_add(&s_res[((i + K / 2) % K) * NT + tn], d);
+                  lds_add(&s_ex[tn], d);
+                  atomicOr(&s_dirty[tn], (1u << ((i + K / 2) % K)) | 0x80000000u);
+                } else {
+                  const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+                  gadd(&p.mbox[(long long)i * N + g], d);
+                  const int sz = lz < 0 ? 0 : (lz >= G::TD ? 2 : 1), sy = ly < 0 ? 0 : (ly >= G::TH ? 2 : 1),
+                            sx = lx < 0 ? 0 : (lx >= G::TW ? 2 : 1);
+                  atomicOr(&s_nbr, 1u << (sz * 9 + sy * 3 + sx));
+                }
               }
             }
           }
@@ -393,11 +437,17 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         const int hv = s_h[hi];
         int m = INF_H;
 #pragma unroll
-        for (int i = 0; i < K; ++i) {
-          if (s_res[i * NT + t] > 0) {
-            const int hn = s_h[hi + (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i]];
-            m = hn < m ? hn : m;
+        for (int c = 0; c < K; c += HOIST) {
+          int hn[HOIST];
+          T rr[HOIST];
+#pragma unroll
+          for (int u = 0; u < HOIST && c + u < K; ++u) {
+            const int i = c + u;
+            hn[u] = s_h[hi + (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i]];
+            rr[u] = s_res[i * NT + t];
           }
+#pragma unroll
+          for (int u = 0; u < HOIST && c + u < K; ++u) m = (rr[u] > 0 && hn[u] < m) ? hn[u] : m;
         }
         if (m >= hv) {
           s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
