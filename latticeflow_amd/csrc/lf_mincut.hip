@@ -75,6 +75,10 @@ struct KP {
   T* ex;
   int* h;
   T* mbox;
+  // mail[t] bit 9 (sz+1) + 3 (sy+1) + (sx+1): the tile at offset (sz,sy,sx)
+  // from t put flow into t's mailbox slots (a hint: the exact passes drain
+  // every mailbox anyway)
+  unsigned* mail;
   int* flag_in;
   int* flag_out;
   unsigned long long* counter;
@@ -236,6 +240,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   __shared__ int s_cnt[2];
   __shared__ int s_tile;
   __shared__ unsigned int s_nbr;  // bit 9 (sz+1) + 3 (sy+1) + (sx+1): neighbouring tile received flow
+  __shared__ unsigned int s_mail;  // sides of this tile whose neighbours sent flow (p.mail)
   static_assert(K <= 31, "dirty mask holds 31 planes");
 
   if (blockIdx.x == 0 && threadIdx.x == 0) *p.wl_cnt_zero = 0;
@@ -248,6 +253,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
       s_cnt[0] = 0;
       s_cnt[1] = 0;
       s_nbr = 0;
+      s_mail = atomicExch(&p.mail[t], 0u);
       atomicAdd(p.tiles, 1ull);
     }
     __syncthreads();
@@ -284,51 +290,60 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         s_dirty[t] = 0;
       }
     }
-    // collect the flow that neighbouring tiles pushed into this tile: plain
-    // loads find the non-empty slots, which are then taken with atomic
-    // exchanges (a sender of this round may still add to them)
+    // collect the flow that neighbouring tiles pushed into this tile, from the
+    // sides the mail bits name: every slot is taken with an atomic exchange
+    // (a sender of this round may still add to it; what it adds later waits
+    // for the next round), all exchanges of a thread in flight at once
+    if (s_mail) {
+      const unsigned mail = s_mail;
+      T add_e[G::NPT];
+      unsigned int dirty[G::NPT];
 #pragma unroll
-    for (int j = 0; j < G::NPT; ++j) {
-      const int t = threadIdx.x + j * NTHREADS;
-      int z, y, x;
-      tile_coords<G>(t, z, y, x);
-      const bool border = z < G::RZ || z >= G::TD - G::RZ || y < G::RY || y >= G::TH - G::RY ||
-                          x < G::RX || x >= G::TW - G::RX;
-      if (!border || z >= ld || y >= lh || x >= lw) continue;
-      const int gz = z0 + z, gy = y0 + y, gx = x0 + x;
-      T add_e = 0;
-      unsigned int dirty = 0;
+      for (int j = 0; j < G::NPT; ++j) {
+        add_e[j] = 0;
+        dirty[j] = 0;
+      }
 #pragma unroll
       for (int c = 0; c < K; c += HOIST) {
-        long long slot[HOIST];
-        T a[HOIST];
+        T a[G::NPT][HOIST];
 #pragma unroll
-        for (int u = 0; u < HOIST && c + u < K; ++u) {
-          const int i = c + u;
-          const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
-          const int lz = z + dz, ly = y + dy, lx = x + dx;
-          const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
-          const bool from_out = !(lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) &&
-                                nz >= 0 && nz < p.D && ny >= 0 && ny < p.H && nx >= 0 && nx < p.W;
-          slot[u] = from_out ? (long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx : -1;
-          const T hint = __builtin_nontemporal_load(&p.mbox[slot[u] < 0 ? 0 : slot[u]]);
-          a[u] = slot[u] < 0 ? (T)0 : hint;
+        for (int j = 0; j < G::NPT; ++j) {
+          int z, y, x;
+          tile_coords<G>(threadIdx.x + j * NTHREADS, z, y, x);
+          const int gz = z0 + z, gy = y0 + y, gx = x0 + x;
+#pragma unroll
+          for (int u = 0; u < HOIST && c + u < K; ++u) {
+            const int i = c + u;
+            const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+            const int lz = z + dz, ly = y + dy, lx = x + dx;
+            const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
+            const int side = (lz < 0 ? 0 : (lz >= G::TD ? 2 : 1)) * 9 + (ly < 0 ? 0 : (ly >= G::TH ? 2 : 1)) * 3 +
+                             (lx < 0 ? 0 : (lx >= G::TW ? 2 : 1));
+            a[j][u] = 0;
+            if (side != 13 && ((mail >> side) & 1u) && z < ld && y < lh && x < lw && nz >= 0 && nz < p.D &&
+                ny >= 0 && ny < p.H && nx >= 0 && nx < p.W)
+              a[j][u] = gtake(&p.mbox[(long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx]);
+          }
         }
 #pragma unroll
-        for (int u = 0; u < HOIST && c + u < K; ++u)
-          if (a[u] != 0) a[u] = gtake(&p.mbox[slot[u]]);
+        for (int j = 0; j < G::NPT; ++j) {
+          const int t = threadIdx.x + j * NTHREADS;
 #pragma unroll
-        for (int u = 0; u < HOIST && c + u < K; ++u) {
-          if (a[u] == 0) continue;
-          const int i = c + u;
-          add_e += a[u];
-          s_res[i * NT + t] += a[u];  // only this thread touches node t's planes while staging
-          dirty |= (1u << i) | 0x80000000u;
+          for (int u = 0; u < HOIST && c + u < K; ++u) {
+            if (a[j][u] == 0) continue;
+            const int i = c + u;
+            add_e[j] += a[j][u];
+            s_res[i * NT + t] += a[j][u];  // only this thread touches node t's planes while staging
+            dirty[j] |= (1u << i) | 0x80000000u;
+          }
         }
       }
-      if (dirty) {
-        s_ex[t] += add_e;
-        s_dirty[t] = dirty;
+#pragma unroll
+      for (int j = 0; j < G::NPT; ++j) {
+        if (!dirty[j]) continue;
+        const int t = threadIdx.x + j * NTHREADS;
+        s_ex[t] += add_e[j];
+        s_dirty[t] = dirty[j];
       }
     }
 #pragma unroll
@@ -488,7 +503,9 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         m &= m - 1;
         const int nz = tz + b / 9 - 1, ny = ty + (b / 3) % 3 - 1, nx = tx + b % 3 - 1;
         if (nz < 0 || nz >= p.ntz || ny < 0 || ny >= p.nty || nx < 0 || nx >= p.ntx) continue;
-        wl_append(p.flag_out, p.wl_out, p.wl_cnt_out, (nz * p.nty + ny) * p.ntx + nx);
+        const int nt = (nz * p.nty + ny) * p.ntx + nx;
+        if (b != 13) atomicOr(&p.mail[nt], 1u << (26 - b));  // seen from nt, this tile is at the mirrored offset
+        wl_append(p.flag_out, p.wl_out, p.wl_cnt_out, nt);
       }
     }
     __syncthreads();
@@ -513,22 +530,27 @@ __global__ __launch_bounds__(NTHREADS) void k_deliver(KP<T> p) {
   if (lx >= G::RX && lx < G::TW - G::RX && ly >= G::RY && ly < G::TH - G::RY && lz >= G::RZ &&
       lz < G::TD - G::RZ)
     return;
-  T add_ex = 0;
+  // every slot of the node read in one batch (no round runs concurrently here)
+  long long slot[K];
+  T a[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
     const int nz = z + dz, ny = y + dy, nx = x + dx;
-    if (nz < 0 || nz >= p.D || ny < 0 || ny >= p.H || nx < 0 || nx >= p.W) continue;
     const int qz = lz + dz, qy = ly + dy, qx = lx + dx;
-    if (qz >= 0 && qz < G::TD && qy >= 0 && qy < G::TH && qx >= 0 && qx < G::TW) continue;
-    const long long w = ((long long)nz * p.H + ny) * p.W + nx;
-    T* slot = &p.mbox[(long long)((i + K / 2) % K) * p.N + w];
-    const T a = *slot;
-    if (a != 0) {
-      *slot = 0;
-      add_ex += a;
-      p.res[(long long)i * p.N + v] += a;
-    }
+    const bool cross = nz >= 0 && nz < p.D && ny >= 0 && ny < p.H && nx >= 0 && nx < p.W &&
+                       !(qz >= 0 && qz < G::TD && qy >= 0 && qy < G::TH && qx >= 0 && qx < G::TW);
+    slot[i] = cross ? (long long)((i + K / 2) % K) * p.N + ((long long)nz * p.H + ny) * p.W + nx : -1;
+    const T m = p.mbox[slot[i] < 0 ? 0 : slot[i]];
+    a[i] = slot[i] < 0 ? (T)0 : m;
+  }
+  T add_ex = 0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    if (a[i] == 0) continue;
+    p.mbox[slot[i]] = 0;
+    add_ex += a[i];
+    p.res[(long long)i * p.N + v] += a[i];
   }
   if (add_ex != 0) p.ex[v] += add_ex;
 }
@@ -624,24 +646,43 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
   const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
   const int x0 = tx * B::TW, y0 = ty * B::TH, z0 = tz * B::TD;
   const int lw = min(B::TW, p.W - x0), lh = min(B::TH, p.H - y0), ld = min(B::TD, p.D - z0);
-  for (int q = threadIdx.x; q < B::NH; q += BTHREADS) {
-    const int hz = q / (B::HH * B::HW), hy = (q / B::HW) % B::HH, hx = q % B::HW;
-    const int gz = z0 + hz - B::RZ, gy = y0 + hy - B::RY, gx = x0 + hx - B::RX;
-    int v = INF_H;
-    if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W)
-      v = hgl[((long long)gz * p.H + gy) * p.W + gx];
-    s_d[q] = v;
+  // labels of the tile and its halo, and the arc masks.  Slots past the
+  // lattice edge load node 0 and discard it, so that a thread's loads are all
+  // in flight at once instead of waiting one by one behind branches.
+  {
+    constexpr int NHI = (B::NH + BTHREADS - 1) / BTHREADS;
+    int hv[NHI];
+    bool hin[NHI];
+#pragma unroll
+    for (int k = 0; k < NHI; ++k) {
+      const int q = threadIdx.x + k * BTHREADS;
+      const int hz = q / (B::HH * B::HW), hy = (q / B::HW) % B::HH, hx = q % B::HW;
+      const int gz = z0 + hz - B::RZ, gy = y0 + hy - B::RY, gx = x0 + hx - B::RX;
+      hin[k] = q < B::NH && gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W;
+      hv[k] = hgl[hin[k] ? ((long long)gz * p.H + gy) * p.W + gx : 0];
+    }
+    MT mv[B::NPT];
+    bool in[B::NPT];
+#pragma unroll
+    for (int j = 0; j < B::NPT; ++j) {
+      const int t = threadIdx.x + j * BTHREADS;
+      const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
+      in[j] = z < ld && y < lh && x < lw;
+      mv[j] = mask[in[j] ? ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x) : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < NHI; ++k) {
+      const int q = threadIdx.x + k * BTHREADS;
+      if (q < B::NH) s_d[q] = hin[k] ? hv[k] : INF_H;
+    }
+#pragma unroll
+    for (int j = 0; j < B::NPT; ++j) {
+      const int t = threadIdx.x + j * BTHREADS;
+      s_m[t] = in[j] ? mv[j] : (MT)0;
+      s_q[t] = 0;
+    }
   }
   int d0[B::NPT];
-#pragma unroll
-  for (int j = 0; j < B::NPT; ++j) {
-    const int t = threadIdx.x + j * BTHREADS;
-    const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
-    MT m = 0;
-    if (z < ld && y < lh && x < lw) m = mask[((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x)];
-    s_m[t] = m;
-    s_q[t] = 0;
-  }
   __syncthreads();
   // pull pass
 #pragma unroll
@@ -761,18 +802,30 @@ __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restr
   const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
   const int x0 = tx * B::TW, y0 = ty * B::TH, z0 = tz * B::TD;
   const int lw = min(B::TW, p.W - x0), lh = min(B::TH, p.H - y0), ld = min(B::TD, p.D - z0);
-  for (int q = threadIdx.x; q < B::NH; q += BTHREADS) {
-    const int hz = q / (B::HH * B::HW), hy = (q / B::HW) % B::HH, hx = q % B::HW;
-    const int gz = z0 + hz - B::RZ, gy = y0 + hy - B::RY, gx = x0 + hx - B::RX;
-    unsigned char r = 0;
-    MT m = 0;
-    if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W) {
-      const long long g = ((long long)gz * p.H + gy) * p.W + gx;
-      r = (unsigned char)reach[g];
-      m = mask[g];
+  // reach bits and arc masks of the tile and its halo, all loads of a thread
+  // in flight at once (slots past the lattice edge read node 0, discarded)
+  {
+    constexpr int NHI = (B::NH + BTHREADS - 1) / BTHREADS;
+    int rv[NHI];
+    MT mv[NHI];
+    bool hin[NHI];
+#pragma unroll
+    for (int k = 0; k < NHI; ++k) {
+      const int q = threadIdx.x + k * BTHREADS;
+      const int hz = q / (B::HH * B::HW), hy = (q / B::HW) % B::HH, hx = q % B::HW;
+      const int gz = z0 + hz - B::RZ, gy = y0 + hy - B::RY, gx = x0 + hx - B::RX;
+      hin[k] = q < B::NH && gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W;
+      const long long g = hin[k] ? ((long long)gz * p.H + gy) * p.W + gx : 0;
+      rv[k] = reach[g];
+      mv[k] = mask[g];
     }
-    s_r[q] = r;
-    s_m[q] = m;
+#pragma unroll
+    for (int k = 0; k < NHI; ++k) {
+      const int q = threadIdx.x + k * BTHREADS;
+      if (q >= B::NH) continue;
+      s_r[q] = hin[k] ? (unsigned char)rv[k] : (unsigned char)0;
+      s_m[q] = hin[k] ? mv[k] : (MT)0;
+    }
   }
   __syncthreads();
   int r0[B::NPT];
@@ -2141,6 +2194,8 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipMalloc(&wl_[0], sizeof(int) * ntiles_));
     LF_HIP(hipMalloc(&wl_[1], sizeof(int) * ntiles_));
     LF_HIP(hipMalloc(&wl_cnt_, 4 * sizeof(unsigned)));
+    LF_HIP(hipMalloc(&mail_, sizeof(unsigned) * ntiles_));
+    LF_HIP(hipMemsetAsync(mail_, 0, sizeof(unsigned) * ntiles_, s_));
     LF_HIP(hipMemsetAsync(wl_cnt_, 0, 4 * sizeof(unsigned), s_));
     {
       // push rounds run as many blocks as the device holds at once
@@ -2175,7 +2230,7 @@ class Engine final : public MinCutEngine {
     hipStreamSynchronize(s_);
     hipFree(red0_); hipFree(alpha0_); hipFree(res_); hipFree(mbox_); hipFree(ex_); hipFree(h_);
     hipFree(mask_); hipFree(reach_); hipFree(flags_[0]); hipFree(flags_[1]); hipFree(bflags_[0]); hipFree(bflags_[1]); hipFree(dev_small_); hipFree(part_);
-    hipFree(planes_); hipFree(wl_[0]); hipFree(wl_[1]); hipFree(wl_cnt_);
+    hipFree(planes_); hipFree(wl_[0]); hipFree(wl_[1]); hipFree(wl_cnt_); hipFree(mail_);
     hipHostFree(host_small_);
     hipFree(pf_r_); hipFree(pf_reg_); hipFree(pf_tab_); hipFree(pf_nreg_); hipFree(pf_cnt_); hipFree(pf_hist_);
     hipFree(b_rects_); hipFree(b_blk_); hipFree(b_flows_);
@@ -2811,7 +2866,7 @@ class Engine final : public MinCutEngine {
  private:
   KP<T> params() const {
     KP<T> p = p_;
-    p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_;
+    p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_; p.mail = mail_;
     p.counter = counter_ptr();
     p.tiles = tiles_ptr();
     p.flag_in = flags_[cur_];
@@ -3008,7 +3063,9 @@ class Engine final : public MinCutEngine {
         ++stats.bfs_launches;
       }, cap);
     }
-    // push rounds start with exactly the tiles holding active nodes
+    // push rounds start with exactly the tiles holding active nodes, and with
+    // empty mailboxes (delivered above)
+    hipLaunchKernelGGL((k_fill<unsigned>), dim3(blocks_for(ntiles_)), dim3(NTHREADS), 0, s_, mail_, 0u, ntiles_);
     fill_flags(flags_[cur_], 0);
     fill_flags(flags_[cur_ ^ 1], 0);
     KP<T> q = p;
@@ -3047,6 +3104,7 @@ class Engine final : public MinCutEngine {
   int* flags_[2] = {nullptr, nullptr};
   int* wl_[2] = {nullptr, nullptr};  // push work lists (tile ids)
   unsigned* wl_cnt_ = nullptr;       // their lengths, rotating over three counters
+  unsigned* mail_ = nullptr;         // per tile: sides that sent it flow
   int wl_k_ = 0;                     // counter index of the next round's input list
   int wl_grid_[3] = {1, 1, 1};       // resident blocks of k_push per tiling
   int cur_ = 0;
