@@ -46,6 +46,7 @@ def main():
     ap.add_argument("--rounds", type=int, default=0)
     ap.add_argument("--iters", type=int, default=0)
     ap.add_argument("--tile-shape", type=int, default=0)
+    ap.add_argument("--relabel-launches", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
@@ -76,8 +77,9 @@ def main():
     shape = w["shape"]
     lf = LatticeFlow(w["kernel"], "LatticeEnergyMinimizer", shape, numerical_precision=1.0,
                      cap_bytes=cap_bytes)
-    if a.rounds or a.iters or a.tile_shape:
-        lf.setTuning(rounds_per_relabel=a.rounds, local_iters=a.iters, tile_shape=a.tile_shape)
+    if a.rounds or a.iters or a.tile_shape or a.relabel_launches:
+        lf.setTuning(rounds_per_relabel=a.rounds, local_iters=a.iters, tile_shape=a.tile_shape,
+                     relabel_launches=a.relabel_launches)
     # inputs resident in HBM before the timed region
     du = torch.from_numpy(np.ascontiguousarray(w["unary"])).cuda()
     dp = torch.from_numpy(np.ascontiguousarray(w["pairwise"])).cuda()
@@ -86,7 +88,7 @@ def main():
     del du, dp
     offs = lf.directions()
     edges = W.n_edges(offs, shape)
-    if not (a.rounds or a.iters or a.tile_shape or a.no_autotune):
+    if not (a.rounds or a.iters or a.tile_shape or a.relabel_launches or a.no_autotune):
         # the library's autotuner picks the tile shape and relabel schedule
         # (untimed; every candidate gives the same flow and cuts)
         lf.autotune(reps=4)
@@ -230,8 +232,9 @@ def bench_batch(a):
     if a.verbose:
         print("generated and ingested %d lattices in %.1fs, canvas %s" % (count, time.time() - t0, bt.canvas()),
               file=sys.stderr, flush=True)
-    if a.rounds or a.iters or a.tile_shape:
-        bt.setTuning(rounds_per_relabel=a.rounds, local_iters=a.iters, tile_shape=a.tile_shape)
+    if a.rounds or a.iters or a.tile_shape or a.relabel_launches:
+        bt.setTuning(rounds_per_relabel=a.rounds, local_iters=a.iters, tile_shape=a.tile_shape,
+                     relabel_launches=a.relabel_launches)
     elif not a.no_autotune:
         bt.autotune(reps=2)
     tuning = bt.getTuning()

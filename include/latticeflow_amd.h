@@ -41,6 +41,10 @@ typedef struct {
   int32_t local_iters;        /* push/relabel iterations per tile per round, in LDS */
   int32_t max_cycles;         /* cap on global-relabel cycles (0 = none) */
   int32_t tile_shape;         /* push-tile shape: 0 = default, 1..3 = the shapes of lf_autotune */
+  int32_t relabel_launches;   /* launches after which a global relabel between push rounds stops
+                                 (0 = default: 2 for min-cut solves, 16 for parametric / TV
+                                 networks); the relabel that decides termination always runs
+                                 to its fixpoint */
 } lf_tuning;
 
 /* Solver statistics of the last lf_run. */

@@ -38,7 +38,8 @@ class LatticeFlowError(RuntimeError):
 
 class lf_tuning(ctypes.Structure):
     _fields_ = [("rounds_per_relabel", ctypes.c_int32), ("local_iters", ctypes.c_int32),
-                ("max_cycles", ctypes.c_int32), ("tile_shape", ctypes.c_int32)]
+                ("max_cycles", ctypes.c_int32), ("tile_shape", ctypes.c_int32),
+                ("relabel_launches", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

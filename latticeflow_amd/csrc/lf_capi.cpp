@@ -267,31 +267,46 @@ int lf_autotune(lf_handle* h, int reps, lf_tuning* best) {
   return guarded([&] {
     if (!h) fail(LF_ERR_BAD_ARG, "null argument");
     if (reps < 1) reps = 1;
-    // candidates: the adaptive schedule and two fixed ones, on every tile shape
-    const lf_tuning sched[] = {{0, 0, 0, 0}, {4, 4, 0, 0}, {8, 2, 0, 0}, {16, 1, 0, 0}};
+    // one untimed warm-up, then the mean of `reps` timed solves (the
+    // schedule of a push-relabel run is not deterministic, so a candidate
+    // that is only sometimes fast must not win on its best run)
+    auto timed = [&](const lf_tuning& t) {
+      if (h->levels) h->eng->pf_run(t);
+      else h->eng->run(t);
+      double tot = 0;
+      for (int r = 0; r < reps; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (h->levels) h->eng->pf_run(t);
+        else h->eng->run(t);
+        tot += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      }
+      return tot;
+    };
+    // stage 1: the adaptive schedule and three fixed ones, on every tile shape
+    const lf_tuning sched[] = {{0, 0, 0, 0, 0}, {4, 4, 0, 0, 0}, {8, 2, 0, 0, 0}, {16, 1, 0, 0, 0}};
     lf_tuning best_t = h->tuning;
     double best_s = 1e300;
     for (int shape = 1; shape <= 3; ++shape)
       for (const lf_tuning& c : sched) {
         lf_tuning t = c;
         t.tile_shape = shape;
-        // one untimed warm-up, then the mean of `reps` timed solves (the
-        // schedule of a push-relabel run is not deterministic, so a candidate
-        // that is only sometimes fast must not win on its best run)
-        if (h->levels) h->eng->pf_run(t);
-        else h->eng->run(t);
-        double tot = 0;
-        for (int r = 0; r < reps; ++r) {
-          const auto t0 = std::chrono::steady_clock::now();
-          if (h->levels) h->eng->pf_run(t);
-          else h->eng->run(t);
-          tot += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        }
-        if (tot < best_s) {
-          best_s = tot;
+        const double s = timed(t);
+        if (s < best_s) {
+          best_s = s;
           best_t = t;
         }
       }
+    // stage 2: how far the relabels between push rounds reach
+    const lf_tuning base = best_t;
+    for (int cap : {2, 4, 8, 16}) {
+      lf_tuning t = base;
+      t.relabel_launches = cap;
+      const double s = timed(t);
+      if (s < best_s) {
+        best_s = s;
+        best_t = t;
+      }
+    }
     h->tuning = best_t;
     if (best) *best = best_t;
   });

@@ -2330,6 +2330,7 @@ class Engine final : public MinCutEngine {
     if (tun.rounds_per_relabel <= 0) tun.rounds_per_relabel = (int)env_double("LF_ROUNDS", 0);
     if (tun.local_iters <= 0) tun.local_iters = (int)env_double("LF_ITERS", 0);
     if (tun.tile_shape <= 0) tun.tile_shape = (int)env_double("LF_TILE_SHAPE", 0);
+    if (tun.relabel_launches <= 0) tun.relabel_launches = (int)env_double("LF_RELABEL_CAP", 0);
     KP<T> p = params();
     // Fixed tuning when given; otherwise the number of push rounds between two
     // exact relabels follows the measured cost ratio of a relabel and a round
@@ -2344,8 +2345,12 @@ class Engine final : public MinCutEngine {
     bool pushed = false;
     long long cycles = 0;
     // Intermediate relabels stop after kRelabelCap launches; the stopping
-    // decision is always taken on an exact (uncapped) relabel.
-    const long long kRelabelCap = (long long)env_double("LF_RELABEL_CAP", 16);
+    // decision is always taken on an exact (uncapped) relabel.  Labels beyond
+    // a short relabel's reach stay INF, which only idles their nodes until a
+    // later relabel: min-cut networks with sinks everywhere measured fastest
+    // at 2 launches (2048^2 segmentation 5.3 -> 2.2 ms, 256^3 blobs 31 -> 8.5
+    // ms), parametric / TV networks, whose flow travels far, at 16.
+    const long long kRelabelCap = tun.relabel_launches > 0 ? tun.relabel_launches : (pf_mode_ ? 16 : 2);
     bool need_exact = false;
     for (;;) {
       const auto tb = clk::now();
@@ -2457,6 +2462,7 @@ class Engine final : public MinCutEngine {
   }
 
   void pf_run_layers(const lf_tuning& tun, int layers) override {
+    PfMode pm{pf_mode_};
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     begin_stats();
@@ -2483,6 +2489,7 @@ class Engine final : public MinCutEngine {
 
   void pf_run_layers_guided(const lf_tuning& tun, int layers, const float* d_pred, float delta, int max_repairs,
                             double pscale, double poff, lf_guided_stats* gs) override {
+    PfMode pm{pf_mode_};
     struct Reset {
       GuideParams& g;
       ~Reset() { g.pred = nullptr; }
@@ -3122,6 +3129,13 @@ class Engine final : public MinCutEngine {
   const bool trace_ = std::getenv("LF_TRACE") != nullptr;
   const bool trace_pf_ = std::getenv("LF_TRACE_PF") != nullptr;
   bool tv_mode_ = false;  // TV network: regions that a mean cut leaves whole are final
+  bool pf_mode_ = false;  // a parametric divide and conquer is running
+  struct PfMode {        // sets pf_mode_ for the scope of a parametric run
+    bool& m;
+    bool old;
+    explicit PfMode(bool& f) : m(f), old(f) { m = true; }
+    ~PfMode() { m = old; }
+  };
   // LF_PF_ASYNC selects the asynchronous divide and conquer for TV networks
   // (same result; on the 1024^2 x 32-lambda path it measured slower than the
   // depth-by-depth order, so that order is the default)

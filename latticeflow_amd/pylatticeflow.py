@@ -191,8 +191,9 @@ class LatticeFlow(object):
     def resetEnergy(self):
         check(lib().lf_reset_energies(self._h))
 
-    def setTuning(self, rounds_per_relabel=0, local_iters=0, max_cycles=0, tile_shape=0):
-        t = _lib.lf_tuning(int(rounds_per_relabel), int(local_iters), int(max_cycles), int(tile_shape))
+    def setTuning(self, rounds_per_relabel=0, local_iters=0, max_cycles=0, tile_shape=0, relabel_launches=0):
+        t = _lib.lf_tuning(int(rounds_per_relabel), int(local_iters), int(max_cycles), int(tile_shape),
+                           int(relabel_launches))
         check(lib().lf_set_tuning(self._h, ctypes.byref(t)))
 
     def getTuning(self):
@@ -331,8 +332,9 @@ class LatticeFlowBatch(object):
     def resetEnergy(self):
         check(lib().lf_reset_energies(self._h))
 
-    def setTuning(self, rounds_per_relabel=0, local_iters=0, max_cycles=0, tile_shape=0):
-        t = _lib.lf_tuning(int(rounds_per_relabel), int(local_iters), int(max_cycles), int(tile_shape))
+    def setTuning(self, rounds_per_relabel=0, local_iters=0, max_cycles=0, tile_shape=0, relabel_launches=0):
+        t = _lib.lf_tuning(int(rounds_per_relabel), int(local_iters), int(max_cycles), int(tile_shape),
+                           int(relabel_launches))
         check(lib().lf_set_tuning(self._h, ctypes.byref(t)))
 
     def getTuning(self):
