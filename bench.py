@@ -25,6 +25,13 @@ METRIC = "Min-cut Medges/s (2048² 8-conn, 256³ 26-conn); TV-path time learned 
 HBM_PEAK_GBS = 8000.0
 
 
+# Tuning of each min-cut workload's bench line: what lf_autotune picks on it
+# (rounds per relabel, local iterations, tile shape, relabel launches), fixed
+# so that the line and its rocprofv3 profile run the same schedule;
+# --autotune runs the search instead.
+DEFAULT_TUNING = {"seg8": (4, 4, 3, 2), "blobs26": (4, 4, 1, 2), "grid4": (0, 0, 3, 0)}
+
+
 def make_workload(name, shape):
     from latticeflow_amd import workloads as W
     if name == "seg8":
@@ -47,7 +54,8 @@ def main():
     ap.add_argument("--iters", type=int, default=0)
     ap.add_argument("--tile-shape", type=int, default=0)
     ap.add_argument("--relabel-launches", type=int, default=0)
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="batch workload: keep the default tuning")
+    ap.add_argument("--autotune", action="store_true", help="single-lattice workloads: run lf_autotune first")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -77,9 +85,13 @@ def main():
     shape = w["shape"]
     lf = LatticeFlow(w["kernel"], "LatticeEnergyMinimizer", shape, numerical_precision=1.0,
                      cap_bytes=cap_bytes)
-    if a.rounds or a.iters or a.tile_shape or a.relabel_launches:
+    explicit = bool(a.rounds or a.iters or a.tile_shape or a.relabel_launches)
+    if explicit:
         lf.setTuning(rounds_per_relabel=a.rounds, local_iters=a.iters, tile_shape=a.tile_shape,
                      relabel_launches=a.relabel_launches)
+    elif not a.autotune and a.workload in DEFAULT_TUNING:
+        r, it, ts, cap = DEFAULT_TUNING[a.workload]
+        lf.setTuning(rounds_per_relabel=r, local_iters=it, tile_shape=ts, relabel_launches=cap)
     # inputs resident in HBM before the timed region
     du = torch.from_numpy(np.ascontiguousarray(w["unary"])).cuda()
     dp = torch.from_numpy(np.ascontiguousarray(w["pairwise"])).cuda()
@@ -88,7 +100,7 @@ def main():
     del du, dp
     offs = lf.directions()
     edges = W.n_edges(offs, shape)
-    if not (a.rounds or a.iters or a.tile_shape or a.relabel_launches or a.no_autotune):
+    if a.autotune and not explicit:
         # the library's autotuner picks the tile shape and relabel schedule
         # (untimed; every candidate gives the same flow and cuts)
         lf.autotune(reps=4)
@@ -168,7 +180,9 @@ def main():
                    "edges": edges, "nodes": int(np.prod(shape)), "seed": w["seed"],
                    "flow": flow, "parity_vs_cpu": parity,
                    "cycles": st["cycles"], "rounds": st["rounds"], "bfs_launches": st["bfs_launches"],
-                   "tuning": tuning},
+                   "tuning": tuning,
+                   "tuning_source": ("explicit" if explicit else "lf_autotune" if a.autotune else
+                                     "fixed (lf_autotune's pick on this workload)")},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

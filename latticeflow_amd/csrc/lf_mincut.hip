@@ -1383,7 +1383,7 @@ __global__ __launch_bounds__(NTHREADS) void k_reparam(int D, int H, int W, long 
 template <int K, typename T>
 __global__ __launch_bounds__(NTHREADS) void k_load(KP<T> p, const long long* __restrict__ red0,
                                                    const long long* __restrict__ alpha0, int* err,
-                                                   long long* __restrict__ part) {
+                                                   long long* __restrict__ part, int zero_mbox) {
   long long mr = 0, ma = 0;
   bool bad = false;
   for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < p.N; v += (long long)gridDim.x * NTHREADS) {
@@ -1395,7 +1395,7 @@ __global__ __launch_bounds__(NTHREADS) void k_load(KP<T> p, const long long* __r
       bad |= a < 0;
       ma = a > ma ? a : ma;
       p.res[(long long)i * p.N + v] = (T)a;
-      p.mbox[(long long)i * p.N + v] = 0;
+      if (zero_mbox) p.mbox[(long long)i * p.N + v] = 0;
     }
     p.ex[v] = (T)(-r);
   }
@@ -2303,7 +2303,10 @@ class Engine final : public MinCutEngine {
     unsigned long long* mx = reinterpret_cast<unsigned long long*>(dev_small_ + 32);
     LF_HIP(hipMemsetAsync(dev_small_, 0, 64, s_));
     const int lb = red_blocks();
-    hipLaunchKernelGGL((k_load<K, T>), dim3(lb), dim3(NTHREADS), 0, s_, p, red0_, alpha0_, err, part_);
+    // a mailbox that every solve since the last clearing delivered is empty
+    hipLaunchKernelGGL((k_load<K, T>), dim3(lb), dim3(NTHREADS), 0, s_, p, red0_, alpha0_, err, part_,
+                       mbox_dirty_ ? 1 : 0);
+    mbox_dirty_ = false;
     hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 0, mx);
     hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 1, mx + 1);
     LF_HIP(hipGetLastError());
@@ -3034,6 +3037,7 @@ class Engine final : public MinCutEngine {
     });
     cur_ ^= 1;
     wl_k_ = (wl_k_ + 1) % 3;
+    mbox_dirty_ = true;
     ++stats.rounds;
   }
 
@@ -3055,6 +3059,7 @@ class Engine final : public MinCutEngine {
         using GG = decltype(g);
         hipLaunchKernelGGL((k_deliver<K, GG, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
       });
+    if (pushed) mbox_dirty_ = false;  // every slot the rounds filled is delivered and cleared
     if (bits_) {
       const long long nw = (long long)p_.D * p_.H * wp_.ntx;  // one wave per plane word
       hipLaunchKernelGGL((k_bits_init<K, T>), dim3((unsigned)((nw * 64 + NTHREADS - 1) / NTHREADS)), dim3(NTHREADS),
@@ -3112,6 +3117,7 @@ class Engine final : public MinCutEngine {
   int* wl_[2] = {nullptr, nullptr};  // push work lists (tile ids)
   unsigned* wl_cnt_ = nullptr;       // their lengths, rotating over three counters
   unsigned* mail_ = nullptr;         // per tile: sides that sent it flow
+  bool mbox_dirty_ = true;           // the mailbox may hold flow (or was never cleared)
   int wl_k_ = 0;                     // counter index of the next round's input list
   int wl_grid_[3] = {1, 1, 1};       // resident blocks of k_push per tiling
   int cur_ = 0;
