@@ -69,6 +69,11 @@ struct Geo {
   static_assert(NT % NTHREADS == 0, "tile must be a multiple of the block");
 };
 
+// 1 bit per direction: the residual-arc mask of a node
+template <int K>
+using MaskT = typename std::conditional<(K <= 8), unsigned char,
+              typename std::conditional<(K <= 16), unsigned short, unsigned int>::type>::type;
+
 template <typename T>
 struct KP {
   T* res;
@@ -79,6 +84,9 @@ struct KP {
   // from t put flow into t's mailbox slots (a hint: the exact passes drain
   // every mailbox anyway)
   unsigned* mail;
+  // MaskT<K> per node, bit i = (res[i][v] > 0), kept current by every kernel
+  // that changes residuals so that a relabel need not rebuild it
+  void* mask;
   int* flag_in;
   int* flag_out;
   unsigned long long* counter;
@@ -490,6 +498,13 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
           if (dm & (1u << i)) p.res[i * N + g] = s_res[i * NT + t];
         p.ex[g] = e;
         p.h[g] = hv;
+        if (dm & ((1u << K) - 1)) {
+          MaskT<K> m = 0;
+#pragma unroll
+          for (int i = 0; i < K; ++i)
+            if (s_res[i * NT + t] > 0) m |= (MaskT<K>)1 << i;
+          reinterpret_cast<MaskT<K>*>(p.mask)[g] = m;
+        }
       }
       if (e > 0 && hv < INF_H) act = true;
     }
@@ -545,14 +560,19 @@ __global__ __launch_bounds__(NTHREADS) void k_deliver(KP<T> p) {
     a[i] = slot[i] < 0 ? (T)0 : m;
   }
   T add_ex = 0;
+  MaskT<K> got = 0;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     if (a[i] == 0) continue;
     p.mbox[slot[i]] = 0;
     add_ex += a[i];
+    got |= (MaskT<K>)1 << i;  // flow arrived along i: the arc back is residual
     p.res[(long long)i * p.N + v] += a[i];
   }
-  if (add_ex != 0) p.ex[v] += add_ex;
+  if (add_ex != 0) {
+    p.ex[v] += add_ex;
+    reinterpret_cast<MaskT<K>*>(p.mask)[v] |= got;
+  }
 }
 
 // ---- exact global relabel -----------------------------------------------------
@@ -567,6 +587,15 @@ __global__ __launch_bounds__(NTHREADS) void k_bfs_init(KP<T> p, MT* __restrict__
   for (int i = 0; i < K; ++i)
     if (p.res[(long long)i * p.N + v] > 0) m |= (MT)1 << i;
   mask[v] = m;
+  p.h[v] = p.ex[v] < 0 ? 0 : INF_H;
+}
+
+// BFS start labels only (the arc masks are current): h[v] = 0 on sink-capable
+// nodes, INF_H elsewhere.
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void k_label_init(KP<T> p) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= p.N) return;
   p.h[v] = p.ex[v] < 0 ? 0 : INF_H;
 }
 
@@ -1389,15 +1418,18 @@ __global__ __launch_bounds__(NTHREADS) void k_load(KP<T> p, const long long* __r
   for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < p.N; v += (long long)gridDim.x * NTHREADS) {
     const long long r = red0[v];
     mr = (r < 0 ? -r : r) > mr ? (r < 0 ? -r : r) : mr;
+    MaskT<K> m = 0;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       const long long a = alpha0[(long long)i * p.N + v];
       bad |= a < 0;
       ma = a > ma ? a : ma;
+      if (a > 0) m |= (MaskT<K>)1 << i;
       p.res[(long long)i * p.N + v] = (T)a;
       if (zero_mbox) p.mbox[(long long)i * p.N + v] = 0;
     }
     p.ex[v] = (T)(-r);
+    reinterpret_cast<MaskT<K>*>(p.mask)[v] = m;
   }
   mr = block_reduce<true>(mr);
   ma = block_reduce<true>(ma);
@@ -1846,6 +1878,7 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_cutarcs(PFP f, KP<T> p) {
   if (v >= f.N) return;
   const int x = (int)(v % p.W), y = (int)((v / p.W) % p.H), z = (int)(v / ((long long)p.W * p.H));
   const int k = f.reg[v];
+  MaskT<K> cut = 0;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     const int nz = z + p.dirs.dz[i], ny = y + p.dirs.dy[i], nx = x + p.dirs.dx[i];
@@ -1853,9 +1886,13 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_cutarcs(PFP f, KP<T> p) {
     const long long w = ((long long)nz * p.H + ny) * p.W + nx;
     if (f.reg[w] != k) {
       T* a = &p.res[(long long)i * p.N + v];
-      if (*a != 0) *a = 0;
+      if (*a != 0) {
+        *a = 0;
+        cut |= (MaskT<K>)1 << i;
+      }
     }
   }
+  if (cut) reinterpret_cast<MaskT<K>*>(p.mask)[v] &= (MaskT<K>)~cut;
 }
 
 __global__ __launch_bounds__(NTHREADS) void k_pf_cut(PFP f, int* __restrict__ out) {
@@ -2307,6 +2344,7 @@ class Engine final : public MinCutEngine {
     hipLaunchKernelGGL((k_load<K, T>), dim3(lb), dim3(NTHREADS), 0, s_, p, red0_, alpha0_, err, part_,
                        mbox_dirty_ ? 1 : 0);
     mbox_dirty_ = false;
+    mask_valid_ = true;  // k_load wrote every node's arc mask
     hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 0, mx);
     hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 1, mx + 1);
     LF_HIP(hipGetLastError());
@@ -2564,6 +2602,7 @@ class Engine final : public MinCutEngine {
       hipLaunchKernelGGL((k_part_mark), dim3(blocks_for(nparts)), dim3(NTHREADS), 0, s_, g, nparts);
       hipLaunchKernelGGL((k_part_newids), dim3(blocks_for(nparts)), dim3(NTHREADS), 0, s_, pf, g, nparts);
       hipLaunchKernelGGL((k_part_restore<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p, g, alpha0_);
+      mask_valid_ = false;  // restored cut arcs: the next relabel rebuilds the masks
       hipLaunchKernelGGL((k_part_relabel), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, g);
       LF_HIP(hipGetLastError());
       if (read_int(pf.err)) {
@@ -2876,7 +2915,7 @@ class Engine final : public MinCutEngine {
  private:
   KP<T> params() const {
     KP<T> p = p_;
-    p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_; p.mail = mail_;
+    p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_; p.mail = mail_; p.mask = mask_;
     p.counter = counter_ptr();
     p.tiles = tiles_ptr();
     p.flag_in = flags_[cur_];
@@ -3069,7 +3108,12 @@ class Engine final : public MinCutEngine {
         ++stats.bfs_launches;
       }, cap);
     } else {
-      hipLaunchKernelGGL((k_bfs_init<K, T, MT>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, mask_);
+      if (mask_valid_) {
+        hipLaunchKernelGGL((k_label_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
+      } else {
+        hipLaunchKernelGGL((k_bfs_init<K, T, MT>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, mask_);
+        mask_valid_ = true;
+      }
       *exact = run_tiled([&](const BP& q) {
         hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, h_);
         ++stats.bfs_launches;
@@ -3118,6 +3162,7 @@ class Engine final : public MinCutEngine {
   unsigned* wl_cnt_ = nullptr;       // their lengths, rotating over three counters
   unsigned* mail_ = nullptr;         // per tile: sides that sent it flow
   bool mbox_dirty_ = true;           // the mailbox may hold flow (or was never cleared)
+  bool mask_valid_ = false;          // mask_ matches res_ (kept by load, push, delivery, cuts)
   int wl_k_ = 0;                     // counter index of the next round's input list
   int wl_grid_[3] = {1, 1, 1};       // resident blocks of k_push per tiling
   int cur_ = 0;
