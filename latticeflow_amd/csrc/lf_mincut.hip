@@ -42,6 +42,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "lf_engine.hpp"
@@ -2274,6 +2276,7 @@ class Engine final : public MinCutEngine {
     hipFree(g_preg_); hipFree(g_parent_); hipFree(g_idmap_); hipFree(g_merged_); hipFree(g_layer_bad_); hipFree(g_val_);
     hipEventDestroy(ev_[0]);
     hipEventDestroy(ev_[1]);
+    for (auto& kv : graphs_) hipGraphExecDestroy(kv.second);
     hipStreamDestroy(s_);
   }
 
@@ -2426,7 +2429,7 @@ class Engine final : public MinCutEngine {
         const unsigned long long tiles0 = fixed_r ? 0 : read_tiles();
         LF_HIP(hipEventRecord(ev_[0], s_));
         const int n = std::min(chunk, R - done_rounds);
-        for (int r = 0; r < n; ++r) launch_push_round(p);
+        push_rounds(p, n);
         LF_HIP(hipGetLastError());
         LF_HIP(hipEventRecord(ev_[1], s_));
         pushed = true;
@@ -2785,7 +2788,7 @@ class Engine final : public MinCutEngine {
         const unsigned long long tiles0 = fixed_r ? 0 : read_tiles();
         LF_HIP(hipEventRecord(ev_[0], s_));
         const int n = std::min(chunk, R - done_rounds);
-        for (int r = 0; r < n; ++r) launch_push_round(p);
+        push_rounds(p, n);
         LF_HIP(hipGetLastError());
         LF_HIP(hipEventRecord(ev_[1], s_));
         pushed = true;
@@ -3080,6 +3083,41 @@ class Engine final : public MinCutEngine {
     ++stats.rounds;
   }
 
+  // n push rounds in a row.  Their launch parameters depend only on the
+  // tiling, the local iterations, the list phase (flag array x list counter,
+  // 2 x 3) and n, so each such chunk is captured once as a hipGraph and
+  // replayed; the graph holds kernel nodes only (LF_NO_GRAPHS=1: plain launches).
+  void push_rounds(const KP<T>& p, int n) {
+    if (!use_graphs_ || n < 2) {
+      for (int r = 0; r < n; ++r) launch_push_round(p);
+      return;
+    }
+    const auto key = std::make_tuple(variant_, p.iters, cur_ + 2 * wl_k_, n);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+      const int cur0 = cur_, k0 = wl_k_;
+      const long long rounds0 = stats.rounds;
+      LF_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+      for (int r = 0; r < n; ++r) launch_push_round(p);
+      hipGraph_t g = nullptr;
+      LF_HIP(hipStreamEndCapture(s_, &g));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      LF_HIP(e);
+      it = graphs_.emplace(key, ex).first;
+      // the captured launches have not run: back to the chunk's first phase
+      cur_ = cur0;
+      wl_k_ = k0;
+      stats.rounds = rounds0;
+    }
+    LF_HIP(hipGraphLaunch(it->second, s_));
+    cur_ ^= n & 1;
+    wl_k_ = (wl_k_ + n) % 3;
+    stats.rounds += n;
+    mbox_dirty_ = true;
+  }
+
   void launch_flood(const BP& q) {
     if (bits_)
       hipLaunchKernelGGL(k_flood_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, reach_);
@@ -3163,6 +3201,8 @@ class Engine final : public MinCutEngine {
   unsigned* mail_ = nullptr;         // per tile: sides that sent it flow
   bool mbox_dirty_ = true;           // the mailbox may hold flow (or was never cleared)
   bool mask_valid_ = false;          // mask_ matches res_ (kept by load, push, delivery, cuts)
+  const bool use_graphs_ = std::getenv("LF_NO_GRAPHS") == nullptr;
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs_;  // push chunks by (tiling, iters, phase, n)
   int wl_k_ = 0;                     // counter index of the next round's input list
   int wl_grid_[3] = {1, 1, 1};       // resident blocks of k_push per tiling
   int cur_ = 0;

@@ -99,3 +99,67 @@ def test_autotune_keeps_results(oracle):
     lf.run()
     assert lf.flow() == flow
     assert np.array_equal(lf.getCut().reshape(-1), smin)
+
+
+@pytest.mark.parametrize("relabel_launches", [1, 2, 4, 16, 64])
+def test_relabel_reach_keeps_results(oracle, relabel_launches):
+    """However far the relabels between push rounds reach, the solve stops only
+    on an exact relabel: flow and both canonical cuts stay bit-exact."""
+    from latticeflow_amd import workloads as W
+    w = W.seg8(seed=11, shape=(257, 381))
+    for tun in ({"relabel_launches": relabel_launches},
+                {"relabel_launches": relabel_launches, "tile_shape": 2, "rounds_per_relabel": 2}):
+        _check(oracle, w["kernel"], w["shape"], w["unary"], w["pairwise"], cap_bytes=4, tuning=tun)
+
+
+def test_repeated_solves_reuse_state(oracle):
+    """Solves on one handle reuse the device state (mailboxes left empty by the
+    previous solve, arc masks kept current): every repeat, across tile shapes,
+    matches the oracle."""
+    from latticeflow_amd import LatticeFlow, workloads as W
+    w = W.blobs26(seed=13, shape=(24, 30, 36))
+    lf = LatticeFlow(w["kernel"], "LatticeEnergyMinimizer", w["shape"], numerical_precision=1.0, cap_bytes=8)
+    lf.addEnergyInt(w["unary"], w["pairwise"])
+    offs = lf.directions()
+    red, alpha = oracle.reparam(w["shape"], offs, w["unary"], w["pairwise"])
+    ref = oracle.prflow(w["shape"], offs, red, alpha)
+    for ts in (1, 3, 2, 1):
+        lf.setTuning(tile_shape=ts)
+        for _ in range(2):
+            lf.run()
+            assert lf.flow() == ref["flow"]
+            assert np.array_equal(lf.getCut("smin").reshape(-1), ref["smin"].astype(np.int32))
+
+
+def test_graph_replays_match_plain_launches(oracle):
+    """Push chunks replayed from captured hipGraphs give the results of plain
+    launches on every replay, with the solver's own eager work (relabels,
+    delivery, flood) between replays."""
+    import os
+    from latticeflow_amd import LatticeFlow, workloads as W
+    w = W.seg8(seed=17, shape=(333, 270))
+    handles = {}
+    for mode in ("graphs", "plain"):
+        old = os.environ.get("LF_NO_GRAPHS")
+        if mode == "plain":
+            os.environ["LF_NO_GRAPHS"] = "1"
+        try:
+            lf = LatticeFlow(w["kernel"], "LatticeEnergyMinimizer", w["shape"], numerical_precision=1.0, cap_bytes=4)
+        finally:
+            if mode == "plain":
+                if old is None:
+                    os.environ.pop("LF_NO_GRAPHS", None)
+                else:
+                    os.environ["LF_NO_GRAPHS"] = old
+        lf.setTuning(rounds_per_relabel=6, local_iters=2, tile_shape=3)
+        lf.addEnergyInt(w["unary"], w["pairwise"])
+        handles[mode] = lf
+    offs = handles["plain"].directions()
+    red, alpha = oracle.reparam(w["shape"], offs, w["unary"], w["pairwise"])
+    ref = oracle.prflow(w["shape"], offs, red, alpha)
+    for rep in range(4):
+        for mode, lf in handles.items():
+            lf.run()
+            assert lf.flow() == ref["flow"], (mode, rep)
+            assert np.array_equal(lf.getCut("smin").reshape(-1), ref["smin"].astype(np.int32)), (mode, rep)
+            assert np.array_equal(lf.getCut("smax").reshape(-1), ref["smax"].astype(np.int32)), (mode, rep)
