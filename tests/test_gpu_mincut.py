@@ -163,3 +163,34 @@ def test_graph_replays_match_plain_launches(oracle):
             assert lf.flow() == ref["flow"], (mode, rep)
             assert np.array_equal(lf.getCut("smin").reshape(-1), ref["smin"].astype(np.int32)), (mode, rep)
             assert np.array_equal(lf.getCut("smax").reshape(-1), ref["smax"].astype(np.int32)), (mode, rep)
+
+
+@pytest.mark.parametrize("kernel,shape", [("Full2d_4", (200, 131)), ("Full2d_8", (130, 257)), ("Full2d_8", (64, 64)),
+                                          ("Full2d_8", (1, 300)), ("Full1d_2", (500,))])
+def test_wave_tile_exact_passes(oracle, kernel, shape):
+    """The wave-tile exact BFS and S_min flood over arc bit planes (selected
+    with LF_WAVE_BFS=1 when the handle is created) give the bit-exact flow and
+    cuts, on lattices with partial tiles and single rows."""
+    import os
+    from latticeflow_amd import LatticeFlow
+    rng = np.random.default_rng(len(shape) * 1000 + shape[0])
+    offs, _ = oracle.edge_order(kernel, shape)
+    un, b = _random_energies(rng, shape, offs, cap=200, wcap=60)
+    old = os.environ.get("LF_WAVE_BFS")
+    os.environ["LF_WAVE_BFS"] = "1"
+    try:
+        lf = LatticeFlow(kernel, "LatticeEnergyMinimizer", shape, numerical_precision=1.0, cap_bytes=4)
+    finally:
+        if old is None:
+            os.environ.pop("LF_WAVE_BFS", None)
+        else:
+            os.environ["LF_WAVE_BFS"] = old
+    lf.addEnergyInt(un, b)
+    red, alpha = oracle.reparam(shape, lf.directions(), un, b)
+    ref = oracle.prflow(shape, lf.directions(), red, alpha)
+    for tun in ({}, {"relabel_launches": 16, "tile_shape": 2}):
+        lf.setTuning(**tun)
+        lf.run()
+        assert lf.flow() == ref["flow"], (kernel, shape, tun)
+        assert np.array_equal(lf.getCut("smin").reshape(-1), ref["smin"].astype(np.int32))
+        assert np.array_equal(lf.getCut("smax").reshape(-1), ref["smax"].astype(np.int32))
