@@ -95,6 +95,9 @@ class MinCutEngine {
 };
 
 std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_bytes);
+// Engines of the wider 1-D / 2-D neighbourhoods (radius 2 to 4) that the main
+// factory does not build itself (lf_mincut_wide.hip); nullptr when none fits.
+std::unique_ptr<MinCutEngine> make_wide_engine(const LatticeGeom& g, int cap_bytes);
 
 // Learned TV predictor (lf_tvnet.hip): parameter count, workspace size,
 // forward pass, loss gradients and one Adam update, all on device pointers.

@@ -3277,6 +3277,7 @@ int max_abs(const std::vector<int>& v) {
 
 }  // namespace
 
+#ifndef LF_MINCUT_WIDE_TU
 std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_bytes) {
   if (cap_bytes != 4 && cap_bytes != 8) fail(LF_ERR_BAD_ARG, "cap_bytes must be 4 or 8");
   const int rz = max_abs(g.dz), ry = max_abs(g.dy), rx = max_abs(g.dx);
@@ -3304,7 +3305,10 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
     if (g.k == 18) return make_k<18, Geo<4, 4, 16, 1, 1, 1>, Geo<2, 8, 16, 1, 1, 1>, Geo<4, 8, 8, 1, 1, 1>, B>(g, cap_bytes);
     if (g.k == 26) return make_k<26, Geo<4, 4, 16, 1, 1, 1>, Geo<2, 8, 16, 1, 1, 1>, Geo<4, 8, 8, 1, 1, 1>, B>(g, cap_bytes);
   }
-  fail(LF_ERR_UNSUPPORTED, "no device path for this kernel size / radius yet");
+  if (auto e = make_wide_engine(g, cap_bytes)) return e;
+  fail(LF_ERR_UNSUPPORTED, "no device path for this kernel: neighbourhoods of 32 or more directions "
+                           "(Star2d_32, Full2d_36, Full2d_48, Full3d_32) are not built");
 }
+#endif
 
 }  // namespace lfamd

@@ -194,3 +194,18 @@ def test_wave_tile_exact_passes(oracle, kernel, shape):
         assert lf.flow() == ref["flow"], (kernel, shape, tun)
         assert np.array_equal(lf.getCut("smin").reshape(-1), ref["smin"].astype(np.int32))
         assert np.array_equal(lf.getCut("smax").reshape(-1), ref["smax"].astype(np.int32))
+
+
+@pytest.mark.parametrize("kernel,nd", [("Full1d_4", 1), ("Full1d_6", 1), ("Full1d_8", 1), ("Full2d_20", 2),
+                                       ("Full2d_24", 2), ("Star2d_24", 2), ("Full2d_28", 2)])
+@pytest.mark.parametrize("cap_bytes", [4, 8])
+def test_wide_neighbourhoods(oracle, kernel, nd, cap_bytes):
+    """Radius 2-4 kernels of the reference family (code_setup.py:71-155) on the
+    device: flow and both canonical cuts bit-exact, every tile shape."""
+    rng = np.random.default_rng(hash((kernel, cap_bytes, 7)) & 0xffff)
+    for trial in range(3):
+        shape = (int(rng.integers(5, 400)),) if nd == 1 else tuple(int(x) for x in rng.integers(5, 70, 2))
+        offs, _ = oracle.edge_order(kernel, shape)
+        un, b = _random_energies(rng, shape, offs, cap=60, wcap=12)
+        print("case", kernel, cap_bytes, shape, flush=True)
+        _check(oracle, kernel, shape, un, b, cap_bytes, tuning={"tile_shape": 1 + trial % 3})
