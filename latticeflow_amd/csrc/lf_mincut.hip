@@ -55,7 +55,7 @@ constexpr int NTHREADS = 256;
 // directions whose LDS operands the push round reads in one batch
 constexpr int HOIST = 8;
 constexpr int INF_H = 1 << 30;
-constexpr int MAXK = 32;
+constexpr int MAXK = 48;
 
 struct DirTab {
   int8_t dz[MAXK], dy[MAXK], dx[MAXK];
@@ -74,7 +74,8 @@ struct Geo {
 // 1 bit per direction: the residual-arc mask of a node
 template <int K>
 using MaskT = typename std::conditional<(K <= 8), unsigned char,
-              typename std::conditional<(K <= 16), unsigned short, unsigned int>::type>::type;
+              typename std::conditional<(K <= 16), unsigned short,
+              typename std::conditional<(K <= 32), unsigned int, unsigned long long>::type>::type>::type;
 
 template <typename T>
 struct KP {
@@ -167,53 +168,6 @@ __device__ inline int tile_of_block(int nblocks) {
   return (b & 7) * per + (b >> 3);
 }
 
-template <class G>
-struct TileCtx {
-  int x0, y0, z0;
-  long long g[G::NPT];   // global index of owned node j, -1 outside the lattice
-  int hi[G::NPT];        // LDS halo index of owned node j
-  unsigned m_lat[G::NPT];  // bit i: neighbour i inside the lattice
-  unsigned m_tile[G::NPT]; // bit i: neighbour i inside this tile
-};
-
-template <int K, class G, typename T>
-__device__ inline void tile_setup(const KP<T>& p, int tile, TileCtx<G>& c) {
-  const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
-  c.x0 = tx * G::TW; c.y0 = ty * G::TH; c.z0 = tz * G::TD;
-#pragma unroll
-  for (int j = 0; j < G::NPT; ++j) {
-    const int t = threadIdx.x + j * NTHREADS;
-    const int z = t / (G::TH * G::TW), y = (t / G::TW) % G::TH, x = t % G::TW;
-    const int gz = c.z0 + z, gy = c.y0 + y, gx = c.x0 + x;
-    const bool valid = gz < p.D && gy < p.H && gx < p.W;
-    c.g[j] = valid ? ((long long)gz * p.H + gy) * p.W + gx : -1;
-    c.hi[j] = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
-    unsigned ml = 0, mt = 0;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
-      const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
-      if (valid && nz >= 0 && nz < p.D && ny >= 0 && ny < p.H && nx >= 0 && nx < p.W) ml |= 1u << i;
-      const int lz = z + dz, ly = y + dy, lx = x + dx;
-      if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) mt |= 1u << i;
-    }
-    c.m_lat[j] = ml;
-    c.m_tile[j] = mt;
-  }
-}
-
-template <class G, typename T>
-__device__ inline void load_halo_h(const KP<T>& p, const TileCtx<G>& c, const int* __restrict__ src, int* s_h) {
-  for (int q = threadIdx.x; q < G::NH; q += NTHREADS) {
-    const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
-    const int gz = c.z0 + hz - G::RZ, gy = c.y0 + hy - G::RY, gx = c.x0 + hx - G::RX;
-    int v = INF_H;
-    if (gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W)
-      v = src[((long long)gz * p.H + gy) * p.W + gx];
-    s_h[q] = v;
-  }
-}
-
 // Flag the neighbouring tile that holds neighbour i of a node at global (gz,gy,gx).
 template <class G, typename T>
 __device__ inline void flag_neighbor_tile(const KP<T>& p, int gz, int gy, int gx, int* flags) {
@@ -246,12 +200,15 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   __shared__ T s_ex[NT];
   __shared__ int s_h[G::NH];
   __shared__ unsigned short s_list[NT];
-  __shared__ unsigned int s_dirty[NT];  // bit i: residual plane i changed; bit 31: excess or label
+  // bit i: residual plane i changed; top bit: excess or label changed
+  using DirtyT = typename std::conditional<(K <= 31), unsigned int, unsigned long long>::type;
+  constexpr DirtyT kTouched = (DirtyT)1 << (8 * sizeof(DirtyT) - 1);
+  __shared__ DirtyT s_dirty[NT];
   __shared__ int s_cnt[2];
   __shared__ int s_tile;
   __shared__ unsigned int s_nbr;  // bit 9 (sz+1) + 3 (sy+1) + (sx+1): neighbouring tile received flow
   __shared__ unsigned int s_mail;  // sides of this tile whose neighbours sent flow (p.mail)
-  static_assert(K <= 31, "dirty mask holds 31 planes");
+  static_assert(K <= 63, "dirty words hold 63 planes");
 
   if (blockIdx.x == 0 && threadIdx.x == 0) *p.wl_cnt_zero = 0;
   const int ntodo = (int)*p.wl_cnt_in;
@@ -307,7 +264,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
     if (s_mail) {
       const unsigned mail = s_mail;
       T add_e[G::NPT];
-      unsigned int dirty[G::NPT];
+      DirtyT dirty[G::NPT];
 #pragma unroll
       for (int j = 0; j < G::NPT; ++j) {
         add_e[j] = 0;
@@ -344,7 +301,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
             const int i = c + u;
             add_e[j] += a[j][u];
             s_res[i * NT + t] += a[j][u];  // only this thread touches node t's planes while staging
-            dirty[j] |= (1u << i) | 0x80000000u;
+            dirty[j] |= ((DirtyT)1 << i) | kTouched;
           }
         }
       }
@@ -405,7 +362,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         // node t's residuals, and a receiver's increase lands on an arc that t
         // does not push along, since both ends cannot be lower than the other)
         T e = e0;
-        unsigned int dirty = 0;
+        DirtyT dirty = 0;
 #pragma unroll
         for (int c = 0; c < K; c += HOIST) {
           int hn[HOIST];
@@ -426,13 +383,13 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
                 const T d = r < e ? r : e;
                 s_res[i * NT + t] = r - d;
                 e -= d;
-                dirty |= 1u << i;
+                dirty |= (DirtyT)1 << i;
                 const int lz = z + dz, ly = y + dy, lx = x + dx;
                 if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) {
                   const int tn = t + (dz * G::TH + dy) * G::TW + dx;
                   lds_add(&s_res[((i + K / 2) % K) * NT + tn], d);
                   lds_add(&s_ex[tn], d);
-                  atomicOr(&s_dirty[tn], (1u << ((i + K / 2) % K)) | 0x80000000u);
+                  atomicOr(&s_dirty[tn], ((DirtyT)1 << ((i + K / 2) % K)) | kTouched);
                 } else {
                   const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
                   gadd(&p.mbox[(long long)i * N + g], d);
@@ -446,7 +403,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         }
         if (e != e0) {
           lds_add(&s_ex[t], e - e0);
-          atomicOr(&s_dirty[t], dirty | 0x80000000u);
+          atomicOr(&s_dirty[t], dirty | kTouched);
         }
       }
       __syncthreads();
@@ -476,7 +433,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         }
         if (m >= hv) {
           s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
-          atomicOr(&s_dirty[t], 0x80000000u);
+          atomicOr(&s_dirty[t], kTouched);
         }
       }
       __syncthreads();
@@ -490,17 +447,17 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
       int z, y, x;
       tile_coords<G>(t, z, y, x);
       if (z >= ld || y >= lh || x >= lw) continue;
-      const unsigned int dm = s_dirty[t];
+      const DirtyT dm = s_dirty[t];
       const T e = s_ex[t];
       const int hv = s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)];
       if (dm) {
         const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
 #pragma unroll
         for (int i = 0; i < K; ++i)
-          if (dm & (1u << i)) p.res[i * N + g] = s_res[i * NT + t];
+          if (dm & ((DirtyT)1 << i)) p.res[i * N + g] = s_res[i * NT + t];
         p.ex[g] = e;
         p.h[g] = hv;
-        if (dm & ((1u << K) - 1)) {
+        if (dm & ~kTouched) {
           MaskT<K> m = 0;
 #pragma unroll
           for (int i = 0; i < K; ++i)
@@ -3263,8 +3220,7 @@ class Engine final : public MinCutEngine {
 
 template <int K, class G0, class G1, class G2, class B>
 std::unique_ptr<MinCutEngine> make_k(const LatticeGeom& g, int cap_bytes) {
-  using MT = typename std::conditional<(K <= 8), unsigned char,
-             typename std::conditional<(K <= 16), unsigned short, unsigned int>::type>::type;
+  using MT = MaskT<K>;
   if (cap_bytes == 4) return std::unique_ptr<MinCutEngine>(new Engine<K, G0, G1, G2, B, int, MT>(g));
   return std::unique_ptr<MinCutEngine>(new Engine<K, G0, G1, G2, B, long long, MT>(g));
 }
@@ -3306,8 +3262,7 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
     if (g.k == 26) return make_k<26, Geo<4, 4, 16, 1, 1, 1>, Geo<2, 8, 16, 1, 1, 1>, Geo<4, 8, 8, 1, 1, 1>, B>(g, cap_bytes);
   }
   if (auto e = make_wide_engine(g, cap_bytes)) return e;
-  fail(LF_ERR_UNSUPPORTED, "no device path for this kernel: neighbourhoods of 32 or more directions "
-                           "(Star2d_32, Full2d_36, Full2d_48, Full3d_32) are not built");
+  fail(LF_ERR_UNSUPPORTED, "no device path for this kernel (more than 48 directions or radius above 4)");
 }
 #endif
 

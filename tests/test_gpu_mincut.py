@@ -197,14 +197,21 @@ def test_wave_tile_exact_passes(oracle, kernel, shape):
 
 
 @pytest.mark.parametrize("kernel,nd", [("Full1d_4", 1), ("Full1d_6", 1), ("Full1d_8", 1), ("Full2d_20", 2),
-                                       ("Full2d_24", 2), ("Star2d_24", 2), ("Full2d_28", 2)])
+                                       ("Full2d_24", 2), ("Star2d_24", 2), ("Full2d_28", 2), ("Star2d_32", 2),
+                                       ("Full2d_36", 2), ("Full2d_48", 2), ("Full3d_32", 3)])
 @pytest.mark.parametrize("cap_bytes", [4, 8])
 def test_wide_neighbourhoods(oracle, kernel, nd, cap_bytes):
-    """Radius 2-4 kernels of the reference family (code_setup.py:71-155) on the
-    device: flow and both canonical cuts bit-exact, every tile shape."""
+    """Radius 2-4 kernels of the reference family (code_setup.py:71-155), up
+    to 48 directions, on the device: flow and both canonical cuts bit-exact,
+    every tile shape."""
     rng = np.random.default_rng(hash((kernel, cap_bytes, 7)) & 0xffff)
     for trial in range(3):
-        shape = (int(rng.integers(5, 400)),) if nd == 1 else tuple(int(x) for x in rng.integers(5, 70, 2))
+        if nd == 1:
+            shape = (int(rng.integers(5, 400)),)
+        elif nd == 2:
+            shape = tuple(int(x) for x in rng.integers(5, 70, 2))
+        else:
+            shape = tuple(int(x) for x in rng.integers(3, 18, 3))
         offs, _ = oracle.edge_order(kernel, shape)
         un, b = _random_energies(rng, shape, offs, cap=60, wcap=12)
         print("case", kernel, cap_bytes, shape, flush=True)
