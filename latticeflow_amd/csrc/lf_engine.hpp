@@ -98,8 +98,10 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
 // Engines of the wider 1-D / 2-D neighbourhoods (radius 2 to 4) that the main
 // factory does not build itself (lf_mincut_wide.hip); nullptr when none fits.
 std::unique_ptr<MinCutEngine> make_wide_engine(const LatticeGeom& g, int cap_bytes);
-// ... and of the neighbourhoods of 32 to 48 directions (lf_mincut_wide32.hip).
+// ... of the radius-3 neighbourhoods of 32 and 36 directions (lf_mincut_wide32.hip)
 std::unique_ptr<MinCutEngine> make_wide32_engine(const LatticeGeom& g, int cap_bytes);
+// ... and of Full2d_48 and Full3d_32 (lf_mincut_wide48.hip).
+std::unique_ptr<MinCutEngine> make_wide48_engine(const LatticeGeom& g, int cap_bytes);
 
 // Learned TV predictor (lf_tvnet.hip): parameter count, workspace size,
 // forward pass, loss gradients and one Adam update, all on device pointers.
