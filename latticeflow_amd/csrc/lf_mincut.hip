@@ -619,7 +619,7 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
   __shared__ MT s_m[B::NT];
   __shared__ unsigned short s_f[2][B::NT];
   __shared__ __align__(4) unsigned char s_q[B::NT];
-  __shared__ int s_n[2];
+  __shared__ int s_n[3];
   __shared__ int s_flag;
   const int tile = tile_of_block(p.nblocks);
   if (tile >= p.ntiles) return;
@@ -628,6 +628,7 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
     p.flag_in[tile] = 0;
     s_n[0] = 0;
     s_n[1] = 0;
+    s_n[2] = 0;
   }
   __syncthreads();
   if (!s_flag) return;
@@ -695,10 +696,14 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
     }
   }
   __syncthreads();
-  int cur = 0;
-  for (;;) {
-    const int n = s_n[cur];
+  // one barrier per step: step k reads list k&1 with count s_n[k%3], appends
+  // to list (k+1)&1 with count s_n[(k+1)%3], and clears s_n[(k+2)%3], which
+  // every thread last read before the previous step's barrier
+  for (int k = 0;; ++k) {
+    const int cur = k & 1, c_in = k % 3, c_out = (k + 1) % 3;
+    const int n = s_n[c_in];
     if (n == 0) break;
+    if (threadIdx.x == 0) s_n[(k + 2) % 3] = 0;
     for (int q = threadIdx.x; q < n; q += BTHREADS) {
       const int v = s_f[cur][q];
       // release v's list flag before reading its label: a later drop of the
@@ -722,13 +727,10 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
           unsigned int* wd = (unsigned int*)&s_q[u & ~3];
           const int sh = (u & 3) * 8;
           const unsigned int old = atomicOr(wd, 1u << sh);
-          if (!((old >> sh) & 0xffu)) s_f[cur ^ 1][atomicAdd(&s_n[cur ^ 1], 1)] = (unsigned short)u;
+          if (!((old >> sh) & 0xffu)) s_f[cur ^ 1][atomicAdd(&s_n[c_out], 1)] = (unsigned short)u;
         }
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s_n[cur] = 0;
-    cur ^= 1;
     __syncthreads();
   }
   bool border = false;
@@ -775,7 +777,7 @@ __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restr
   __shared__ __align__(4) unsigned char s_r[(B::NH + 3) & ~3];
   __shared__ MT s_m[B::NH];
   __shared__ unsigned short s_f[2][B::NT];
-  __shared__ int s_n[2];
+  __shared__ int s_n[3];
   __shared__ int s_flag;
   const int tile = tile_of_block(p.nblocks);
   if (tile >= p.ntiles) return;
@@ -784,6 +786,7 @@ __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restr
     p.flag_in[tile] = 0;
     s_n[0] = 0;
     s_n[1] = 0;
+    s_n[2] = 0;
   }
   __syncthreads();
   if (!s_flag) return;
@@ -844,10 +847,14 @@ __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restr
     if (r0[j]) s_f[0][atomicAdd(&s_n[0], 1)] = (unsigned short)t;
   }
   __syncthreads();
-  int cur = 0;
-  for (;;) {
-    const int n = s_n[cur];
+  // one barrier per step: step k reads list k&1 with count s_n[k%3], appends
+  // to list (k+1)&1 with count s_n[(k+1)%3], and clears s_n[(k+2)%3], which
+  // every thread last read before the previous step's barrier
+  for (int k = 0;; ++k) {
+    const int cur = k & 1, c_in = k % 3, c_out = (k + 1) % 3;
+    const int n = s_n[c_in];
     if (n == 0) break;
+    if (threadIdx.x == 0) s_n[(k + 2) % 3] = 0;
     for (int q = threadIdx.x; q < n; q += BTHREADS) {
       const int v = s_f[cur][q];
       const int hv = bhalo<B>(v);
@@ -866,12 +873,9 @@ __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restr
         const int sh = (hw & 3) * 8;
         const unsigned int old = atomicOr(wd, 1u << sh);
         if (!((old >> sh) & 0xffu))
-          s_f[cur ^ 1][atomicAdd(&s_n[cur ^ 1], 1)] = (unsigned short)(v + (dz * B::TH + dy) * B::TW + dx);
+          s_f[cur ^ 1][atomicAdd(&s_n[c_out], 1)] = (unsigned short)(v + (dz * B::TH + dy) * B::TW + dx);
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s_n[cur] = 0;
-    cur ^= 1;
     __syncthreads();
   }
   bool border = false;
