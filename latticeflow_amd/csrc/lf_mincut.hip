@@ -615,10 +615,14 @@ __device__ inline void flag_around(const BP& p, int tile) {
 // fixpoint over all tiles is the exact BFS distance.
 template <int K, class B, typename MT>
 __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restrict__ mask, int* __restrict__ hgl) {
+  // a list holds up to half the tile; a step whose list overflowed walks the
+  // listed-node bits instead (every claimed node has its bit set), so the
+  // lists cost half the LDS and more tiles are resident per CU
+  constexpr int LCAP = B::NT / 2;
   __shared__ int s_d[B::NH];
   __shared__ MT s_m[B::NT];
-  __shared__ unsigned short s_f[2][B::NT];
-  __shared__ __align__(4) unsigned char s_q[B::NT];
+  __shared__ unsigned short s_f[2][LCAP];
+  __shared__ unsigned int s_q[B::NT / 32];
   __shared__ int s_n[3];
   __shared__ int s_flag;
   const int tile = tile_of_block(p.nblocks);
@@ -668,8 +672,8 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
     for (int j = 0; j < B::NPT; ++j) {
       const int t = threadIdx.x + j * BTHREADS;
       s_m[t] = in[j] ? mv[j] : (MT)0;
-      s_q[t] = 0;
     }
+    for (int w = threadIdx.x; w < B::NT / 32; w += BTHREADS) s_q[w] = 0;
   }
   int d0[B::NPT];
   __syncthreads();
@@ -691,8 +695,9 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
       }
     if (best < dv) {
       atomicMin(&s_d[hi], best);
-      s_q[t] = 1;
-      s_f[0][atomicAdd(&s_n[0], 1)] = (unsigned short)t;
+      atomicOr(&s_q[t >> 5], 1u << (t & 31));
+      const int pos = atomicAdd(&s_n[0], 1);
+      if (pos < LCAP) s_f[0][pos] = (unsigned short)t;
     }
   }
   __syncthreads();
@@ -704,11 +709,22 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
     const int n = s_n[c_in];
     if (n == 0) break;
     if (threadIdx.x == 0) s_n[(k + 2) % 3] = 0;
-    for (int q = threadIdx.x; q < n; q += BTHREADS) {
-      const int v = s_f[cur][q];
-      // release v's list flag before reading its label: a later drop of the
+    // n counts every claim of the previous step, also those past LCAP
+    const bool scan = n > LCAP;
+    const int lim = scan ? B::NT : n;
+    for (int q = threadIdx.x; q < lim; q += BTHREADS) {
+      int v;
+      if (scan) {
+        // a bit set during this step is a claim for the next one: taking it
+        // now as well only repeats an idempotent relaxation
+        v = q;
+        if (!((s_q[v >> 5] >> (v & 31)) & 1u)) continue;
+      } else {
+        v = s_f[cur][q];
+      }
+      // release v's list bit before reading its label: a later drop of the
       // label then re-lists v instead of being lost
-      atomicAnd((unsigned int*)&s_q[v & ~3], ~(0xffu << ((v & 3) * 8)));
+      atomicAnd(&s_q[v >> 5], ~(1u << (v & 31)));
       __threadfence_block();
       const int hv = bhalo<B>(v);
       const int nd = s_d[hv] + 1;
@@ -723,11 +739,12 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
         if (!(s_m[u] & ((MT)1 << ((j + K / 2) % K)))) continue;
         const int hu = hv + (dz * B::HH + dy) * B::HW + dx;
         if (s_d[hu] > nd && atomicMin(&s_d[hu], nd) > nd) {
-          // claim u for the next list (one byte flag, set through its word)
-          unsigned int* wd = (unsigned int*)&s_q[u & ~3];
-          const int sh = (u & 3) * 8;
-          const unsigned int old = atomicOr(wd, 1u << sh);
-          if (!((old >> sh) & 0xffu)) s_f[cur ^ 1][atomicAdd(&s_n[c_out], 1)] = (unsigned short)u;
+          // claim u for the next list
+          const unsigned int bit = 1u << (u & 31);
+          if (!(atomicOr(&s_q[u >> 5], bit) & bit)) {
+            const int pos = atomicAdd(&s_n[c_out], 1);
+            if (pos < LCAP) s_f[cur ^ 1][pos] = (unsigned short)u;
+          }
         }
       }
     }
@@ -2249,7 +2266,7 @@ class Engine final : public MinCutEngine {
   // else G0.
   int variant_for(const lf_tuning& tun) const {
     if (tun.tile_shape >= 1 && tun.tile_shape <= 3) return tun.tile_shape - 1;
-    return (sizeof(T) == 8 && B::TD == 1) ? 1 : 0;
+    return 0;
   }
 
   void set_grid(int v) {
@@ -3243,7 +3260,7 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
   const int rz = max_abs(g.dz), ry = max_abs(g.dy), rx = max_abs(g.dx);
   const bool is3d = g.ndim == 3;
   if (!is3d && rz == 0 && ry <= 1 && rx <= 1) {
-    using G0 = Geo<1, 16, 64, 0, 1, 1>;
+    using G0 = Geo<1, 16, 32, 0, 1, 1>;
     using G1 = Geo<1, 8, 64, 0, 1, 1>;
     using G2 = Geo<1, 32, 32, 0, 1, 1>;
     using B = BGeo<1, 64, 64, 0, 1, 1>;
@@ -3252,7 +3269,7 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
     if (g.k == 8) return make_k<8, G0, G1, G2, B>(g, cap_bytes);
   }
   if (!is3d && rz == 0 && ry <= 2 && rx <= 2) {
-    using G0 = Geo<1, 16, 64, 0, 2, 2>;
+    using G0 = Geo<1, 16, 32, 0, 2, 2>;
     using G1 = Geo<1, 8, 64, 0, 2, 2>;
     using G2 = Geo<1, 32, 32, 0, 2, 2>;
     using B = BGeo<1, 64, 64, 0, 2, 2>;
