@@ -87,6 +87,9 @@ struct KP {
   // from t put flow into t's mailbox slots (a hint: the exact passes drain
   // every mailbox anyway)
   unsigned* mail;
+  // dmail[t]: the same side bits, kept until the delivery pass, which reads
+  // only the slots of those sides and then clears them
+  unsigned* dmail;
   // MaskT<K> per node, bit i = (res[i][v] > 0), kept current by every kernel
   // that changes residuals so that a relabel need not rebuild it
   void* mask;
@@ -478,7 +481,10 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         const int nz = tz + b / 9 - 1, ny = ty + (b / 3) % 3 - 1, nx = tx + b % 3 - 1;
         if (nz < 0 || nz >= p.ntz || ny < 0 || ny >= p.nty || nx < 0 || nx >= p.ntx) continue;
         const int nt = (nz * p.nty + ny) * p.ntx + nx;
-        if (b != 13) atomicOr(&p.mail[nt], 1u << (26 - b));  // seen from nt, this tile is at the mirrored offset
+        if (b != 13) {  // seen from nt, this tile is at the mirrored offset
+          atomicOr(&p.mail[nt], 1u << (26 - b));
+          atomicOr(&p.dmail[nt], 1u << (26 - b));
+        }
         wl_append(p.flag_out, p.wl_out, p.wl_cnt_out, nt);
       }
     }
@@ -504,6 +510,10 @@ __global__ __launch_bounds__(NTHREADS) void k_deliver(KP<T> p) {
   if (lx >= G::RX && lx < G::TW - G::RX && ly >= G::RY && ly < G::TH - G::RY && lz >= G::RZ &&
       lz < G::TD - G::RZ)
     return;
+  // only sides that sent this tile flow since the last delivery hold slots
+  // with flow (every sender sets the bit after its rounds)
+  const unsigned dm = p.dmail[((z / G::TD) * p.nty + y / G::TH) * p.ntx + x / G::TW];
+  if (!dm) return;
   // every slot of the node read in one batch (no round runs concurrently here)
   long long slot[K];
   T a[K];
@@ -512,8 +522,10 @@ __global__ __launch_bounds__(NTHREADS) void k_deliver(KP<T> p) {
     const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
     const int nz = z + dz, ny = y + dy, nx = x + dx;
     const int qz = lz + dz, qy = ly + dy, qx = lx + dx;
-    const bool cross = nz >= 0 && nz < p.D && ny >= 0 && ny < p.H && nx >= 0 && nx < p.W &&
-                       !(qz >= 0 && qz < G::TD && qy >= 0 && qy < G::TH && qx >= 0 && qx < G::TW);
+    const int side = (qz < 0 ? 0 : (qz >= G::TD ? 2 : 1)) * 9 + (qy < 0 ? 0 : (qy >= G::TH ? 2 : 1)) * 3 +
+                     (qx < 0 ? 0 : (qx >= G::TW ? 2 : 1));
+    const bool cross = nz >= 0 && nz < p.D && ny >= 0 && ny < p.H && nx >= 0 && nx < p.W && side != 13 &&
+                       ((dm >> side) & 1u);
     slot[i] = cross ? (long long)((i + K / 2) % K) * p.N + ((long long)nz * p.H + ny) * p.W + nx : -1;
     const T m = p.mbox[slot[i] < 0 ? 0 : slot[i]];
     a[i] = slot[i] < 0 ? (T)0 : m;
@@ -1395,7 +1407,45 @@ __global__ __launch_bounds__(NTHREADS) void k_load(KP<T> p, const long long* __r
                                                    long long* __restrict__ part, int zero_mbox) {
   long long mr = 0, ma = 0;
   bool bad = false;
-  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < p.N; v += (long long)gridDim.x * NTHREADS) {
+  const long long tid = (long long)blockIdx.x * NTHREADS + threadIdx.x, step = (long long)gridDim.x * NTHREADS;
+  // two nodes per thread with 16-byte loads when every plane starts 16-byte
+  // aligned (N even; the planes are separate allocations)
+  using T2 = typename std::conditional<sizeof(T) == 8, longlong2, int2>::type;
+  const long long npair = (p.N & 1) ? 0 : p.N / 2;
+  for (long long w = tid; w < npair; w += step) {
+    const long long v = 2 * w;
+    const longlong2 r = *reinterpret_cast<const longlong2*>(red0 + v);
+    const long long r0 = r.x < 0 ? -r.x : r.x, r1 = r.y < 0 ? -r.y : r.y;
+    mr = r0 > mr ? r0 : mr;
+    mr = r1 > mr ? r1 : mr;
+    MaskT<K> m0 = 0, m1 = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const longlong2 a = *reinterpret_cast<const longlong2*>(alpha0 + (long long)i * p.N + v);
+      bad |= (a.x < 0) | (a.y < 0);
+      ma = a.x > ma ? a.x : ma;
+      ma = a.y > ma ? a.y : ma;
+      if (a.x > 0) m0 |= (MaskT<K>)1 << i;
+      if (a.y > 0) m1 |= (MaskT<K>)1 << i;
+      T2 o;
+      o.x = (T)a.x;
+      o.y = (T)a.y;
+      *reinterpret_cast<T2*>(p.res + (long long)i * p.N + v) = o;
+      if (zero_mbox) {
+        T2 z;
+        z.x = 0;
+        z.y = 0;
+        *reinterpret_cast<T2*>(p.mbox + (long long)i * p.N + v) = z;
+      }
+    }
+    T2 e;
+    e.x = (T)(-r.x);
+    e.y = (T)(-r.y);
+    *reinterpret_cast<T2*>(p.ex + v) = e;
+    reinterpret_cast<MaskT<K>*>(p.mask)[v] = m0;
+    reinterpret_cast<MaskT<K>*>(p.mask)[v + 1] = m1;
+  }
+  for (long long v = npair ? p.N : tid; v < p.N; v += step) {
     const long long r = red0[v];
     mr = (r < 0 ? -r : r) > mr ? (r < 0 ? -r : r) : mr;
     MaskT<K> m = 0;
@@ -2213,6 +2263,8 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipMalloc(&wl_cnt_, 4 * sizeof(unsigned)));
     LF_HIP(hipMalloc(&mail_, sizeof(unsigned) * ntiles_));
     LF_HIP(hipMemsetAsync(mail_, 0, sizeof(unsigned) * ntiles_, s_));
+    LF_HIP(hipMalloc(&dmail_, sizeof(unsigned) * ntiles_));
+    LF_HIP(hipMemsetAsync(dmail_, 0, sizeof(unsigned) * ntiles_, s_));
     LF_HIP(hipMemsetAsync(wl_cnt_, 0, 4 * sizeof(unsigned), s_));
     {
       // push rounds run as many blocks as the device holds at once
@@ -2247,7 +2299,7 @@ class Engine final : public MinCutEngine {
     hipStreamSynchronize(s_);
     hipFree(red0_); hipFree(alpha0_); hipFree(res_); hipFree(mbox_); hipFree(ex_); hipFree(h_);
     hipFree(mask_); hipFree(reach_); hipFree(flags_[0]); hipFree(flags_[1]); hipFree(bflags_[0]); hipFree(bflags_[1]); hipFree(dev_small_); hipFree(part_);
-    hipFree(planes_); hipFree(wl_[0]); hipFree(wl_[1]); hipFree(wl_cnt_); hipFree(mail_);
+    hipFree(planes_); hipFree(wl_[0]); hipFree(wl_[1]); hipFree(wl_cnt_); hipFree(mail_); hipFree(dmail_);
     hipHostFree(host_small_);
     hipFree(pf_r_); hipFree(pf_reg_); hipFree(pf_tab_); hipFree(pf_nreg_); hipFree(pf_cnt_); hipFree(pf_hist_);
     hipFree(b_rects_); hipFree(b_blk_); hipFree(b_flows_);
@@ -2324,6 +2376,8 @@ class Engine final : public MinCutEngine {
     // a mailbox that every solve since the last clearing delivered is empty
     hipLaunchKernelGGL((k_load<K, T>), dim3(lb), dim3(NTHREADS), 0, s_, p, red0_, alpha0_, err, part_,
                        mbox_dirty_ ? 1 : 0);
+    if (mbox_dirty_)
+      hipLaunchKernelGGL((k_fill<unsigned>), dim3(blocks_for(ntiles_)), dim3(NTHREADS), 0, s_, dmail_, 0u, ntiles_);
     mbox_dirty_ = false;
     mask_valid_ = true;  // k_load wrote every node's arc mask
     hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 0, mx);
@@ -2896,7 +2950,7 @@ class Engine final : public MinCutEngine {
  private:
   KP<T> params() const {
     KP<T> p = p_;
-    p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_; p.mail = mail_; p.mask = mask_;
+    p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_; p.mail = mail_; p.dmail = dmail_; p.mask = mask_;
     p.counter = counter_ptr();
     p.tiles = tiles_ptr();
     p.flag_in = flags_[cur_];
@@ -3114,6 +3168,8 @@ class Engine final : public MinCutEngine {
         using GG = decltype(g);
         hipLaunchKernelGGL((k_deliver<K, GG, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
       });
+    if (pushed)
+      hipLaunchKernelGGL((k_fill<unsigned>), dim3(blocks_for(ntiles_)), dim3(NTHREADS), 0, s_, dmail_, 0u, ntiles_);
     if (pushed) mbox_dirty_ = false;  // every slot the rounds filled is delivered and cleared
     if (bits_) {
       const long long nw = (long long)p_.D * p_.H * wp_.ntx;  // one wave per plane word
@@ -3177,6 +3233,7 @@ class Engine final : public MinCutEngine {
   int* wl_[2] = {nullptr, nullptr};  // push work lists (tile ids)
   unsigned* wl_cnt_ = nullptr;       // their lengths, rotating over three counters
   unsigned* mail_ = nullptr;         // per tile: sides that sent it flow
+  unsigned* dmail_ = nullptr;        // per tile: sides that sent it flow since the last delivery
   bool mbox_dirty_ = true;           // the mailbox may hold flow (or was never cleared)
   bool mask_valid_ = false;          // mask_ matches res_ (kept by load, push, delivery, cuts)
   const bool use_graphs_ = std::getenv("LF_NO_GRAPHS") == nullptr;
