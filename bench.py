@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # (rounds per relabel, local iterations, tile shape, relabel launches), fixed
 # so that the line and its rocprofv3 profile run the same schedule;
 # --autotune runs the search instead.
-DEFAULT_TUNING = {"seg8": (4, 4, 3, 2), "blobs26": (4, 4, 1, 2), "grid4": (0, 0, 3, 0)}
+DEFAULT_TUNING = {"seg8": (4, 4, 1, 2), "blobs26": (4, 4, 1, 2), "grid4": (0, 0, 3, 0)}
 
 
 def make_workload(name, shape):
@@ -182,7 +182,7 @@ def main():
                    "cycles": st["cycles"], "rounds": st["rounds"], "bfs_launches": st["bfs_launches"],
                    "tuning": tuning,
                    "tuning_source": ("explicit" if explicit else "lf_autotune" if a.autotune else
-                                     "fixed (lf_autotune's pick on this workload)")},
+                                     "fixed (the fastest of lf_autotune's candidates measured on this workload)")},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
