@@ -2427,12 +2427,13 @@ class Engine final : public MinCutEngine {
     // at 2 launches (2048^2 segmentation 5.3 -> 2.2 ms, 256^3 blobs 31 -> 8.5
     // ms), parametric / TV networks, whose flow travels far, at 16.
     const long long kRelabelCap = tun.relabel_launches > 0 ? tun.relabel_launches : (pf_mode_ ? 16 : 2);
-    bool need_exact = false;
+    bool need_exact = false, resume = false;
     for (;;) {
       const auto tb = clk::now();
       const long long bl0 = stats.bfs_launches;
       bool exact = false;
-      const long long active = global_relabel(p, pushed, need_exact ? 0 : kRelabelCap, &exact);
+      const long long active = global_relabel(p, pushed, need_exact ? 0 : kRelabelCap, &exact, resume);
+      resume = false;
       pushed = false;  // the mailbox was delivered
       const double t_relabel = std::chrono::duration<double>(clk::now() - tb).count();
       stats.bfs_seconds += t_relabel;
@@ -2447,10 +2448,14 @@ class Engine final : public MinCutEngine {
       }
       if (active == 0) {
         if (exact) break;
-        need_exact = true;  // confirm with an uncapped relabel before stopping
+        // confirm before stopping: carry the capped pass on to its fixpoint
+        need_exact = true;
+        resume = true;
         continue;
       }
-      need_exact = false;
+      // once a confirmation finds only a few active nodes left, the last
+      // cycles relabel exactly (a capped relabel would need a confirmation again)
+      need_exact = need_exact && active * 1024 < N_;
       if (tun.max_cycles > 0 && cycles > tun.max_cycles)
         fail(LF_ERR_STATE, "max_cycles reached before the flow became maximum");
       p.iters = tun.local_iters > 0 ? tun.local_iters : (active * 64 < N_ ? 16 : 4);
@@ -3068,11 +3073,15 @@ class Engine final : public MinCutEngine {
 
   // Repeat a tiled exact pass until no tile border changes.  Launches go in
   // batches; only the last launch of a batch is checked, launches over
-  // converged tiles exit at their flag test.
+  // converged tiles exit at their flag test.  resume: carry on from the tile
+  // flags a capped pass left (its labels and flags are those of an uncapped
+  // pass after the same launches).
   template <class F>
-  bool run_tiled(F&& launch, long long cap = 0) {
-    fill_flags(bflags_[bcur_], 1, bntiles_);
-    fill_flags(bflags_[bcur_ ^ 1], 0, bntiles_);
+  bool run_tiled(F&& launch, long long cap = 0, bool resume = false) {
+    if (!resume) {
+      fill_flags(bflags_[bcur_], 1, bntiles_);
+      fill_flags(bflags_[bcur_ ^ 1], 0, bntiles_);
+    }
     long long launched = 0;
     for (int batch = 2;; batch = std::min(batch * 2, 8)) {
       if (cap > 0 && launched >= cap) return false;  // stopped before the fixpoint
@@ -3161,8 +3170,10 @@ class Engine final : public MinCutEngine {
   // number of active nodes.  With cap > 0 the BFS may stop after cap launches:
   // the labels are then lengths of real residual paths (upper bounds of the
   // distances) and unreached nodes are INF, which only steers pushes; *exact
-  // tells whether the fixpoint was reached.
-  long long global_relabel(KP<T> p, bool pushed, long long cap, bool* exact) {
+  // tells whether the fixpoint was reached.  resume: no push round since a
+  // capped relabel, whose pass is carried on to the fixpoint.
+  long long global_relabel(KP<T> p, bool pushed, long long cap, bool* exact, bool resume = false) {
+    if (resume) pushed = false;
     if (pushed)
       with_geo([&](auto g) {
         using GG = decltype(g);
@@ -3173,14 +3184,17 @@ class Engine final : public MinCutEngine {
     if (pushed) mbox_dirty_ = false;  // every slot the rounds filled is delivered and cleared
     if (bits_) {
       const long long nw = (long long)p_.D * p_.H * wp_.ntx;  // one wave per plane word
-      hipLaunchKernelGGL((k_bits_init<K, T>), dim3((unsigned)((nw * 64 + NTHREADS - 1) / NTHREADS)), dim3(NTHREADS),
-                         0, s_, p, wp_, planes_);
+      if (!resume)
+        hipLaunchKernelGGL((k_bits_init<K, T>), dim3((unsigned)((nw * 64 + NTHREADS - 1) / NTHREADS)),
+                           dim3(NTHREADS), 0, s_, p, wp_, planes_);
       *exact = run_tiled([&](const BP& q) {
         hipLaunchKernelGGL(k_bfs_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, h_);
         ++stats.bfs_launches;
-      }, cap);
+      }, cap, resume);
     } else {
-      if (mask_valid_) {
+      if (resume) {
+        // labels and tile flags as the capped pass left them
+      } else if (mask_valid_) {
         hipLaunchKernelGGL((k_label_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
       } else {
         hipLaunchKernelGGL((k_bfs_init<K, T, MT>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, mask_);
@@ -3189,7 +3203,7 @@ class Engine final : public MinCutEngine {
       *exact = run_tiled([&](const BP& q) {
         hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, h_);
         ++stats.bfs_launches;
-      }, cap);
+      }, cap, resume);
     }
     // push rounds start with exactly the tiles holding active nodes, and with
     // empty mailboxes (delivered above)
