@@ -2427,7 +2427,7 @@ class Engine final : public MinCutEngine {
     // at 2 launches (2048^2 segmentation 5.3 -> 2.2 ms, 256^3 blobs 31 -> 8.5
     // ms), parametric / TV networks, whose flow travels far, at 16.
     const long long kRelabelCap = tun.relabel_launches > 0 ? tun.relabel_launches : (pf_mode_ ? 16 : 2);
-    bool need_exact = false, resume = false;
+    bool need_exact = false, resume = false, push_timed_later = false;
     for (;;) {
       const auto tb = clk::now();
       const long long bl0 = stats.bfs_launches;
@@ -2435,7 +2435,15 @@ class Engine final : public MinCutEngine {
       const long long active = global_relabel(p, pushed, need_exact ? 0 : kRelabelCap, &exact, resume);
       resume = false;
       pushed = false;  // the mailbox was delivered
-      const double t_relabel = std::chrono::duration<double>(clk::now() - tb).count();
+      double t_relabel = std::chrono::duration<double>(clk::now() - tb).count();
+      if (push_timed_later) {
+        // the relabel synchronised the stream: the rounds before it are done
+        float ms = 0;
+        LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        stats.push_seconds += ms * 1e-3;
+        t_relabel = std::max(0.0, t_relabel - ms * 1e-3);
+        push_timed_later = false;
+      }
       stats.bfs_seconds += t_relabel;
       ++stats.cycles;
       ++cycles;
@@ -2470,6 +2478,13 @@ class Engine final : public MinCutEngine {
         LF_HIP(hipGetLastError());
         LF_HIP(hipEventRecord(ev_[1], s_));
         pushed = true;
+        if (fixed_r && done_rounds + n >= R) {
+          // the cycle's last chunk of a fixed schedule: its time is read after
+          // the next relabel's synchronisation, so the host does not wait here
+          done_rounds += n;
+          push_timed_later = true;
+          break;
+        }
         LF_HIP(hipEventSynchronize(ev_[1]));
         float ms = 0;
         LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
@@ -2965,6 +2980,7 @@ class Engine final : public MinCutEngine {
   unsigned long long* counter_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 16); }
   int red_blocks() const { return (int)std::min<long long>(RED_BLOCKS, std::max<long long>(1, blocks_for(N_))); }
   unsigned long long* tiles_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 24); }
+  unsigned long long* active_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 48); }
   void zero_counter() {
     hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, counter_ptr(), 0ull, 1LL);
   }
@@ -3075,9 +3091,11 @@ class Engine final : public MinCutEngine {
   // batches; only the last launch of a batch is checked, launches over
   // converged tiles exit at their flag test.  resume: carry on from the tile
   // flags a capped pass left (its labels and flags are those of an uncapped
-  // pass after the same launches).
+  // pass after the same launches).  deferred (capped passes): the batch that
+  // reaches the cap is not checked here; *deferred tells the caller to read
+  // the border counter with its own next read.
   template <class F>
-  bool run_tiled(F&& launch, long long cap = 0, bool resume = false) {
+  bool run_tiled(F&& launch, long long cap = 0, bool resume = false, bool* deferred = nullptr) {
     if (!resume) {
       fill_flags(bflags_[bcur_], 1, bntiles_);
       fill_flags(bflags_[bcur_ ^ 1], 0, bntiles_);
@@ -3096,6 +3114,10 @@ class Engine final : public MinCutEngine {
         ++launched;
       }
       LF_HIP(hipGetLastError());
+      if (deferred && cap > 0 && launched >= cap) {
+        *deferred = true;
+        return false;
+      }
       if (read_counter() == 0) return true;
       // labels / reach bits only move one way, so the fixpoint is reached;
       // a pass that does not get there within this many launches is a bug
@@ -3174,6 +3196,7 @@ class Engine final : public MinCutEngine {
   // capped relabel, whose pass is carried on to the fixpoint.
   long long global_relabel(KP<T> p, bool pushed, long long cap, bool* exact, bool resume = false) {
     if (resume) pushed = false;
+    bool deferred = false;
     if (pushed)
       with_geo([&](auto g) {
         using GG = decltype(g);
@@ -3190,7 +3213,7 @@ class Engine final : public MinCutEngine {
       *exact = run_tiled([&](const BP& q) {
         hipLaunchKernelGGL(k_bfs_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, h_);
         ++stats.bfs_launches;
-      }, cap, resume);
+      }, cap, resume, &deferred);
     } else {
       if (resume) {
         // labels and tile flags as the capped pass left them
@@ -3203,7 +3226,7 @@ class Engine final : public MinCutEngine {
       *exact = run_tiled([&](const BP& q) {
         hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, h_);
         ++stats.bfs_launches;
-      }, cap, resume);
+      }, cap, resume, &deferred);
     }
     // push rounds start with exactly the tiles holding active nodes, and with
     // empty mailboxes (delivered above)
@@ -3220,9 +3243,15 @@ class Engine final : public MinCutEngine {
     hipLaunchKernelGGL((k_fill<unsigned>), dim3(1), dim3(NTHREADS), 0, s_, wl_cnt_, 0u, 3LL);
     hipLaunchKernelGGL((k_flags_to_list), dim3(blocks_for(p_.ntiles)), dim3(NTHREADS), 0, s_, flags_[cur_],
                        p_.ntiles, wl_[cur_], wl_cnt_ + wl_k_);
-    hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
+    hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, active_ptr());
     LF_HIP(hipGetLastError());
-    return (long long)read_counter();
+    // one read: the border counter of the last BFS launch (slot 0) up to the
+    // active count (slot 4)
+    LF_HIP(hipMemcpyAsync(host_small_, counter_ptr(), 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    const unsigned long long* hs = reinterpret_cast<const unsigned long long*>(host_small_);
+    if (deferred) *exact = hs[0] == 0;
+    return (long long)hs[4];
   }
 
   LatticeGeom g_;
