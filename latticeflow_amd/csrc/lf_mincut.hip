@@ -2470,8 +2470,8 @@ class Engine final : public MinCutEngine {
       const int chunk = fixed_r ? R : 4;
       int done_rounds = 0;
       double t_push = 0;
+      unsigned long long tiles0 = tiles_seen_;  // no round since the relabel read it
       while (done_rounds < R) {
-        const unsigned long long tiles0 = fixed_r ? 0 : read_tiles();
         LF_HIP(hipEventRecord(ev_[0], s_));
         const int n = std::min(chunk, R - done_rounds);
         push_rounds(p, n);
@@ -2485,12 +2485,20 @@ class Engine final : public MinCutEngine {
           push_timed_later = true;
           break;
         }
-        LF_HIP(hipEventSynchronize(ev_[1]));
+        // one synchronisation: the tile counter (adaptive schedule) or the event
+        unsigned long long tiles1 = 0;
+        if (fixed_r)
+          LF_HIP(hipEventSynchronize(ev_[1]));
+        else
+          tiles1 = read_tiles();
         float ms = 0;
         LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
         t_push += ms * 1e-3;
         done_rounds += n;
-        if (!fixed_r && read_tiles() == tiles0) break;  // no tile had work
+        if (!fixed_r) {
+          if (tiles1 == tiles0) break;  // no tile had work
+          tiles0 = tiles1;
+        }
       }
       stats.push_seconds += t_push;
       if (!fixed_r) {
@@ -2836,20 +2844,28 @@ class Engine final : public MinCutEngine {
       const int chunk = fixed_r ? R : 4;
       int done_rounds = 0;
       double t_push = 0;
+      unsigned long long tiles0 = tiles_seen_;  // no round since the relabel read it
       while (done_rounds < R) {
-        const unsigned long long tiles0 = fixed_r ? 0 : read_tiles();
         LF_HIP(hipEventRecord(ev_[0], s_));
         const int n = std::min(chunk, R - done_rounds);
         push_rounds(p, n);
         LF_HIP(hipGetLastError());
         LF_HIP(hipEventRecord(ev_[1], s_));
         pushed = true;
-        LF_HIP(hipEventSynchronize(ev_[1]));
+        // one synchronisation: the tile counter (adaptive schedule) or the event
+        unsigned long long tiles1 = 0;
+        if (fixed_r)
+          LF_HIP(hipEventSynchronize(ev_[1]));
+        else
+          tiles1 = read_tiles();
         float ms = 0;
         LF_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
         t_push += ms * 1e-3;
         done_rounds += n;
-        if (!fixed_r && read_tiles() == tiles0) break;  // no tile had work
+        if (!fixed_r) {
+          if (tiles1 == tiles0) break;  // no tile had work
+          tiles0 = tiles1;
+        }
       }
       stats.push_seconds += t_push;
       if (!fixed_r) {
@@ -2980,6 +2996,7 @@ class Engine final : public MinCutEngine {
   unsigned long long* counter_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 16); }
   int red_blocks() const { return (int)std::min<long long>(RED_BLOCKS, std::max<long long>(1, blocks_for(N_))); }
   unsigned long long* tiles_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 24); }
+  unsigned long long tiles_seen_ = 0;  // processed-tile counter as the last relabel read it
   unsigned long long* active_ptr() const { return reinterpret_cast<unsigned long long*>(dev_small_ + 48); }
   void zero_counter() {
     hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, counter_ptr(), 0ull, 1LL);
@@ -3251,6 +3268,7 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipStreamSynchronize(s_));
     const unsigned long long* hs = reinterpret_cast<const unsigned long long*>(host_small_);
     if (deferred) *exact = hs[0] == 0;
+    tiles_seen_ = hs[1];
     return (long long)hs[4];
   }
 
