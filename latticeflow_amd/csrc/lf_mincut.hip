@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -2467,7 +2468,9 @@ class Engine final : public MinCutEngine {
       if (tun.max_cycles > 0 && cycles > tun.max_cycles)
         fail(LF_ERR_STATE, "max_cycles reached before the flow became maximum");
       p.iters = tun.local_iters > 0 ? tun.local_iters : (active * 64 < N_ ? 16 : 4);
-      const int chunk = fixed_r ? R : 4;
+      // adaptive schedule: chunks of 4, 8, then 16 rounds while tiles have
+      // work (one host read per chunk; R is a multiple of 4)
+      int chunk = fixed_r ? R : 4;
       int done_rounds = 0;
       double t_push = 0;
       unsigned long long tiles0 = tiles_seen_;  // no round since the relabel read it
@@ -2498,13 +2501,14 @@ class Engine final : public MinCutEngine {
         if (!fixed_r) {
           if (tiles1 == tiles0) break;  // no tile had work
           tiles0 = tiles1;
+          chunk = std::min(chunk * 2, 16);
         }
       }
       stats.push_seconds += t_push;
       if (!fixed_r) {
         const double per_round = t_push / std::max(done_rounds, 1);
         const double want = per_round > 0 ? t_relabel / per_round : 4.0;
-        R = (int)std::max(4.0, std::min(256.0, want));
+        R = 4 * (int)std::max(1.0, std::min(64.0, std::round(want / 4)));
       }
     }
     const auto tf = clk::now();
@@ -2841,7 +2845,9 @@ class Engine final : public MinCutEngine {
       }
       if (act == 0) break;
       p.iters = tun.local_iters > 0 ? tun.local_iters : (act * 64 < N_ ? 16 : 4);
-      const int chunk = fixed_r ? R : 4;
+      // adaptive schedule: chunks of 4, 8, then 16 rounds while tiles have
+      // work (one host read per chunk; R is a multiple of 4)
+      int chunk = fixed_r ? R : 4;
       int done_rounds = 0;
       double t_push = 0;
       unsigned long long tiles0 = tiles_seen_;  // no round since the relabel read it
@@ -2865,13 +2871,14 @@ class Engine final : public MinCutEngine {
         if (!fixed_r) {
           if (tiles1 == tiles0) break;  // no tile had work
           tiles0 = tiles1;
+          chunk = std::min(chunk * 2, 16);
         }
       }
       stats.push_seconds += t_push;
       if (!fixed_r) {
         const double per_round = t_push / std::max(done_rounds, 1);
         const double want = per_round > 0 ? t_relabel / per_round : 4.0;
-        R = (int)std::max(4.0, std::min(256.0, want));
+        R = 4 * (int)std::max(1.0, std::min(64.0, std::round(want / 4)));
       }
     }
   }

@@ -216,3 +216,28 @@ def test_wide_neighbourhoods(oracle, kernel, nd, cap_bytes):
         un, b = _random_energies(rng, shape, offs, cap=60, wcap=12)
         print("case", kernel, cap_bytes, shape, flush=True)
         _check(oracle, kernel, shape, un, b, cap_bytes, tuning={"tile_shape": 1 + trial % 3})
+
+
+def test_bfs_list_overflow(oracle):
+    """A sink on every (even, even) node of an 8-connected lattice: the first
+    exact-BFS step of a 64x64 tile lists three quarters of its nodes, more than
+    a list holds (half a tile), so that step walks the listed-node bits; flow
+    and both canonical cuts stay bit-exact."""
+    shape = (150, 203)
+    offs, _ = oracle.edge_order("Full2d_8", shape)
+    rng = np.random.default_rng(23)
+    n = shape[0] * shape[1]
+    yy, xx = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    sink = ((yy % 2 == 0) & (xx % 2 == 0)).reshape(-1)
+    kp = len(offs) // 2
+    b = np.zeros((n, kp, 4), np.int64)
+    b[:, :, 1] = rng.integers(0, 20, (n, kp))
+    b[:, :, 2] = rng.integers(0, 20, (n, kp))
+    un = np.zeros((n, 2), np.int64)
+    un[sink, 1] = 1000 + rng.integers(0, 50, int(sink.sum()))
+    un[~sink, 0] = 300 + rng.integers(0, 50, int((~sink).sum()))
+    red, _ = oracle.reparam(shape, offs, un, b)
+    # sink-capable nodes (reduced unary > 0) exactly on the pattern
+    assert np.array_equal(red > 0, sink) and np.all(red[~sink] < 0)
+    for tun in ({}, {"relabel_launches": 1, "tile_shape": 2}):
+        _check(oracle, "Full2d_8", shape, un, b, cap_bytes=4, tuning=tun)
