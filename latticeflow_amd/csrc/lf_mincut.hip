@@ -39,6 +39,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <initializer_list>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -1475,6 +1476,23 @@ template <typename V>
 __global__ void k_fill(V* a, V val, long long n) {
   const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
   if (v < n) a[v] = val;
+}
+
+// Up to six 32-bit fills in one launch (the per-relabel resets of tile flags,
+// mail bits and list counters are each a few thousand words).
+struct FillSeg {
+  unsigned* p;
+  unsigned v;
+  long long n;
+};
+struct FillSet {
+  FillSeg s[6];
+  int ns;
+};
+__global__ __launch_bounds__(NTHREADS) void k_fill_multi(FillSet f) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  for (int i = 0; i < f.ns; ++i)
+    if (v < f.s[i].n) f.s[i].p[v] = f.s[i].v;
 }
 
 // ---- parametric divide and conquer (ParametricFlowSolver, pf_solver.hpp:33-208) ----
@@ -3106,6 +3124,18 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipStreamSynchronize(s_));
     return *reinterpret_cast<unsigned long long*>(host_small_);
   }
+  void fill_many(std::initializer_list<FillSeg> segs) {
+    FillSet f{};
+    long long mx = 0;
+    for (const FillSeg& g : segs) {
+      if (!g.p || g.n <= 0) continue;
+      if (f.ns == 6) fail(LF_ERR_STATE, "fill_many: more than six segments");
+      f.s[f.ns++] = g;
+      mx = std::max(mx, g.n);
+    }
+    if (f.ns) hipLaunchKernelGGL(k_fill_multi, dim3(blocks_for(mx)), dim3(NTHREADS), 0, s_, f);
+  }
+  static unsigned* u32(void* ptr) { return reinterpret_cast<unsigned*>(ptr); }
   void fill_flags(int* f, int v, long long n = -1) {
     if (n < 0) n = ntiles_;
     hipLaunchKernelGGL((k_fill<int>), dim3(blocks_for(n)), dim3(NTHREADS), 0, s_, f, v, n);
@@ -3120,10 +3150,7 @@ class Engine final : public MinCutEngine {
   // the border counter with its own next read.
   template <class F>
   bool run_tiled(F&& launch, long long cap = 0, bool resume = false, bool* deferred = nullptr) {
-    if (!resume) {
-      fill_flags(bflags_[bcur_], 1, bntiles_);
-      fill_flags(bflags_[bcur_ ^ 1], 0, bntiles_);
-    }
+    if (!resume) fill_many({{u32(bflags_[bcur_]), 1u, bntiles_}, {u32(bflags_[bcur_ ^ 1]), 0u, bntiles_}});
     long long launched = 0;
     for (int batch = 2;; batch = std::min(batch * 2, 8)) {
       if (cap > 0 && launched >= cap) return false;  // stopped before the fixpoint
@@ -3226,8 +3253,6 @@ class Engine final : public MinCutEngine {
         using GG = decltype(g);
         hipLaunchKernelGGL((k_deliver<K, GG, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
       });
-    if (pushed)
-      hipLaunchKernelGGL((k_fill<unsigned>), dim3(blocks_for(ntiles_)), dim3(NTHREADS), 0, s_, dmail_, 0u, ntiles_);
     if (pushed) mbox_dirty_ = false;  // every slot the rounds filled is delivered and cleared
     if (bits_) {
       const long long nw = (long long)p_.D * p_.H * wp_.ntx;  // one wave per plane word
@@ -3253,10 +3278,12 @@ class Engine final : public MinCutEngine {
       }, cap, resume, &deferred);
     }
     // push rounds start with exactly the tiles holding active nodes, and with
-    // empty mailboxes (delivered above)
-    hipLaunchKernelGGL((k_fill<unsigned>), dim3(blocks_for(ntiles_)), dim3(NTHREADS), 0, s_, mail_, 0u, ntiles_);
-    fill_flags(flags_[cur_], 0);
-    fill_flags(flags_[cur_ ^ 1], 0);
+    // empty mailboxes (delivered above: no mail or delivery bits left)
+    fill_many({{mail_, 0u, ntiles_},
+               {pushed ? dmail_ : nullptr, 0u, ntiles_},
+               {u32(flags_[cur_]), 0u, ntiles_},
+               {u32(flags_[cur_ ^ 1]), 0u, ntiles_},
+               {wl_cnt_, 0u, 3}});
     KP<T> q = p;
     q.flag_out = flags_[cur_];
     with_geo([&](auto g) {
@@ -3264,7 +3291,6 @@ class Engine final : public MinCutEngine {
       hipLaunchKernelGGL((k_count_active<GG, T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, q, part_);
     });
     // the first round's work list: the flagged tiles
-    hipLaunchKernelGGL((k_fill<unsigned>), dim3(1), dim3(NTHREADS), 0, s_, wl_cnt_, 0u, 3LL);
     hipLaunchKernelGGL((k_flags_to_list), dim3(blocks_for(p_.ntiles)), dim3(NTHREADS), 0, s_, flags_[cur_],
                        p_.ntiles, wl_[cur_], wl_cnt_ + wl_k_);
     hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, active_ptr());
