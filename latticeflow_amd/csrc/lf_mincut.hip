@@ -779,10 +779,9 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
   if (__syncthreads_or(border) && threadIdx.x == 0) flag_around(p, tile);
 }
 
-// count active nodes (ex > 0, h < INF) into *total (cleared beforehand) and
-// flag their tiles for the push rounds
+// count active nodes (ex > 0, h < INF) and flag their tiles for the push rounds
 template <class G, typename T>
-__global__ __launch_bounds__(NTHREADS) void k_count_active(KP<T> p, unsigned long long* __restrict__ total) {
+__global__ __launch_bounds__(NTHREADS) void k_count_active(KP<T> p, long long* __restrict__ part) {
   long long cnt = 0;
   for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < p.N; v += (long long)gridDim.x * NTHREADS) {
     if (p.ex[v] > 0 && p.h[v] < INF_H) {
@@ -792,7 +791,7 @@ __global__ __launch_bounds__(NTHREADS) void k_count_active(KP<T> p, unsigned lon
     }
   }
   cnt = block_reduce<false>(cnt);
-  if (threadIdx.x == 0 && cnt) atomicAdd(total, (unsigned long long)cnt);
+  if (threadIdx.x == 0) part[blockIdx.x] = cnt;
 }
 
 // ---- S_min: flood fill from excess nodes over residual arcs --------------------
@@ -3284,17 +3283,17 @@ class Engine final : public MinCutEngine {
                {pushed ? dmail_ : nullptr, 0u, ntiles_},
                {u32(flags_[cur_]), 0u, ntiles_},
                {u32(flags_[cur_ ^ 1]), 0u, ntiles_},
-               {wl_cnt_, 0u, 3},
-               {u32(active_ptr()), 0u, 2}});
+               {wl_cnt_, 0u, 3}});
     KP<T> q = p;
     q.flag_out = flags_[cur_];
     with_geo([&](auto g) {
       using GG = decltype(g);
-      hipLaunchKernelGGL((k_count_active<GG, T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, q, active_ptr());
+      hipLaunchKernelGGL((k_count_active<GG, T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, q, part_);
     });
     // the first round's work list: the flagged tiles
     hipLaunchKernelGGL((k_flags_to_list), dim3(blocks_for(p_.ntiles)), dim3(NTHREADS), 0, s_, flags_[cur_],
                        p_.ntiles, wl_[cur_], wl_cnt_ + wl_k_);
+    hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, active_ptr());
     LF_HIP(hipGetLastError());
     // one read: the border counter of the last BFS launch (slot 0) up to the
     // active count (slot 4)
