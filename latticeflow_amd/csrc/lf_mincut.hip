@@ -504,9 +504,7 @@ __global__ __launch_bounds__(NTHREADS) void k_flags_to_list(const int* __restric
 // ---- mailbox delivery (before any exact pass) --------------------------------
 
 template <int K, class G, typename T>
-__global__ __launch_bounds__(NTHREADS) void k_deliver(KP<T> p) {
-  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
-  if (v >= p.N) return;
+__device__ inline void deliver_node(const KP<T>& p, long long v) {
   const int x = (int)(v % p.W), y = (int)((v / p.W) % p.H), z = (int)(v / ((long long)p.W * p.H));
   const int lx = x % G::TW, ly = y % G::TH, lz = z % G::TD;
   if (lx >= G::RX && lx < G::TW - G::RX && ly >= G::RY && ly < G::TH - G::RY && lz >= G::RZ &&
@@ -546,6 +544,17 @@ __global__ __launch_bounds__(NTHREADS) void k_deliver(KP<T> p) {
     p.ex[v] += add_ex;
     reinterpret_cast<MaskT<K>*>(p.mask)[v] |= got;
   }
+}
+
+// Delivery pass; with init_labels it also sets the BFS start labels (h = 0 on
+// sink-capable nodes, INF_H elsewhere) from the delivered excess, which saves
+// the separate label pass when the arc masks are current.
+template <int K, class G, typename T>
+__global__ __launch_bounds__(NTHREADS) void k_deliver(KP<T> p, int init_labels) {
+  const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+  if (v >= p.N) return;
+  deliver_node<K, G, T>(p, v);
+  if (init_labels) p.h[v] = p.ex[v] < 0 ? 0 : INF_H;
 }
 
 // ---- exact global relabel -----------------------------------------------------
@@ -3248,10 +3257,13 @@ class Engine final : public MinCutEngine {
   long long global_relabel(KP<T> p, bool pushed, long long cap, bool* exact, bool resume = false) {
     if (resume) pushed = false;
     bool deferred = false;
+    // the delivery pass also sets the start labels when the masks are current
+    const bool fused_init = pushed && !bits_ && mask_valid_;
     if (pushed)
       with_geo([&](auto g) {
         using GG = decltype(g);
-        hipLaunchKernelGGL((k_deliver<K, GG, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
+        hipLaunchKernelGGL((k_deliver<K, GG, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p,
+                           fused_init ? 1 : 0);
       });
     if (pushed) mbox_dirty_ = false;  // every slot the rounds filled is delivered and cleared
     if (bits_) {
@@ -3267,7 +3279,7 @@ class Engine final : public MinCutEngine {
       if (resume) {
         // labels and tile flags as the capped pass left them
       } else if (mask_valid_) {
-        hipLaunchKernelGGL((k_label_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
+        if (!fused_init) hipLaunchKernelGGL((k_label_init<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p);
       } else {
         hipLaunchKernelGGL((k_bfs_init<K, T, MT>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, mask_);
         mask_valid_ = true;
