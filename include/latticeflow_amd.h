@@ -116,8 +116,9 @@ int lf_directions(const lf_handle *h, int32_t *offsets);
  * unary[2*n + {0,1}] = E0, E1; binary[n*4*(k/2) + 4*e + {0..3}] = E00,E01,E10,E11.
  * Host pointers; accumulates like the reference (a second call adds). */
 int lf_add_energy_potentials(lf_handle *h, const int64_t *unary, const int64_t *binary);
-/* The same from device-resident arrays (no host copy), ordered on `stream`
- * (a hipStream_t; NULL = the handle's stream). */
+/* The same from device-resident arrays (no host copy).  `stream` (a
+ * hipStream_t; NULL = the null stream) is the stream that wrote d_unary /
+ * d_binary: the library waits for the work queued on it before reading them. */
 int lf_add_energy_potentials_device(lf_handle *h, const int64_t *d_unary,
                                     const int64_t *d_binary, void *stream);
 /* Clear all accumulated energies (a fresh lattice). */
@@ -146,7 +147,8 @@ int lf_get_cut(const lf_handle *h, int32_t *out, int which);
 /* The max-flow value of the last run in raw (doubled) units
  * (network_flow_node.hpp:234-240); the reference computes it but does not expose it. */
 int lf_get_flow(const lf_handle *h, int64_t *flow);
-/* Device pointer variant of getCut for device-resident consumers. */
+/* Device pointer variant of getCut for device-resident consumers: work queued
+ * on `stream` (NULL = the null stream) after this call sees the cut. */
 int lf_get_cut_device(const lf_handle *h, int32_t *d_out, int which, void *stream);
 /* LatticeLevelReductions: the final value r of every node (raw units) after the
  * divide and conquer, i.e. reduction + offset with all offsets reset to 0
@@ -206,10 +208,12 @@ int lf_tv_path(const char *kernel, int64_t nx, int64_t ny, const double *f, int6
  * between two regions is taken as saturated in the order of the predictions
  * and checked once the regions are solved; regions joined by a failing arc
  * are merged and solved again (after max_repairs checks, whole lambdas), so
- * R is the same solution as lf_tv_path's for any prediction. */
+ * R is the same solution as lf_tv_path's for any prediction.  `stream` (NULL =
+ * the null stream) is the stream that wrote d_pred: the call waits for it
+ * before reading the predictions. */
 int lf_tv_path_guided(const char *kernel, int64_t nx, int64_t ny, const double *f, int64_t n_lambda,
                       const double *lambdas, const float *d_pred, double delta, int max_repairs, double *R,
-                      lf_stats *stats, lf_guided_stats *gstats);
+                      lf_stats *stats, lf_guided_stats *gstats, void *stream);
 
 /* ---- learned TV predictor ---------------------------------------------------
  * No reference counterpart: a 3-layer 3x3 convolutional net (16 channels, BF16

@@ -115,7 +115,7 @@ _SIG = {
     "lf_tv_single": (_I, [_CP, _I64, _I64, _P, ctypes.c_double, _P, ctypes.POINTER(lf_stats)]),
     "lf_tv_path": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.POINTER(lf_stats)]),
     "lf_tv_path_guided": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.c_double, _I, _P,
-                               ctypes.POINTER(lf_stats), ctypes.POINTER(lf_guided_stats)]),
+                               ctypes.POINTER(lf_stats), ctypes.POINTER(lf_guided_stats), _P]),
 }
 
 _lib = None

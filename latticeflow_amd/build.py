@@ -19,15 +19,29 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False, jobs: int = 8) -> str:
-    """Compile every source to an object, then link the shared library."""
+def _newest_header() -> float:
+    heads = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    heads.append(os.path.join(HERE, "..", "include", "latticeflow_amd.h"))
+    return max(os.path.getmtime(h) for h in heads if os.path.exists(h))
+
+
+def build(verbose: bool = False, jobs: int = 8, incremental: bool = False) -> str:
+    """Compile every source to an object, then link the shared library.
+
+    ``incremental``: skip sources whose object is newer than the source and
+    every header (a development shortcut; the default rebuilds everything)."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs = []
     objs = []
+    head_t = _newest_header()
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
+        if incremental and os.path.exists(obj):
+            ot = os.path.getmtime(obj)
+            if ot > os.path.getmtime(os.path.join(CSRC, src)) and ot > head_t:
+                continue
         lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
         cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
@@ -51,4 +65,4 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

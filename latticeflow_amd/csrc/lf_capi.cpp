@@ -750,9 +750,11 @@ int lf_tv_path(const char* kernel, int64_t nx, int64_t ny, const double* f, int6
 
 int lf_tv_path_guided(const char* kernel, int64_t nx, int64_t ny, const double* f, int64_t n_lambda,
                       const double* lambdas, const float* d_pred, double delta, int max_repairs, double* R,
-                      lf_stats* stats, lf_guided_stats* gstats) {
+                      lf_stats* stats, lf_guided_stats* gstats, void* stream) {
   return guarded([&] {
     if (!d_pred) fail(LF_ERR_BAD_ARG, "null prediction");
+    // the predictions are complete before any stream of this call reads them
+    LF_HIP(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
     if (!(delta >= 0)) fail(LF_ERR_BAD_ARG, "delta must be >= 0");
     if (max_repairs < 0) fail(LF_ERR_BAD_ARG, "max_repairs must be >= 0");
     tv_path_impl(kernel, nx, ny, f, n_lambda, lambdas, R, stats, d_pred, delta, max_repairs, gstats);

@@ -2374,13 +2374,7 @@ class Engine final : public MinCutEngine {
   }
 
   void add_energies_device(const int64_t* du, const int64_t* db, hipStream_t s) override {
-    if (s && s != s_) {
-      hipEvent_t ev;
-      LF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      LF_HIP(hipEventRecord(ev, s));
-      LF_HIP(hipStreamWaitEvent(s_, ev, 0));
-      LF_HIP(hipEventDestroy(ev));
-    }
+    wait_for(s);
     int* err = reinterpret_cast<int*>(dev_small_);
     LF_HIP(hipMemsetAsync(err, 0, sizeof(int), s_));
     hipLaunchKernelGGL((k_reparam<K>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p_.D, p_.H, p_.W, N_,
@@ -2962,13 +2956,7 @@ class Engine final : public MinCutEngine {
 
   void batch_add_energies_device(int i, const int64_t* du, const int64_t* db, hipStream_t s) override {
     if (i < 0 || i >= (int)b_rects_host_.size()) fail(LF_ERR_OUT_OF_RANGE, "batch lattice index out of range");
-    if (s && s != s_) {
-      hipEvent_t ev;
-      LF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      LF_HIP(hipEventRecord(ev, s));
-      LF_HIP(hipStreamWaitEvent(s_, ev, 0));
-      LF_HIP(hipEventDestroy(ev));
-    }
+    wait_for(s);
     const BatchRect& R = b_rects_host_[i];
     int* err = reinterpret_cast<int*>(dev_small_);
     LF_HIP(hipMemsetAsync(err, 0, sizeof(int), s_));
@@ -3008,13 +2996,7 @@ class Engine final : public MinCutEngine {
     KP<T> p = params();
     hipLaunchKernelGGL((k_cut<T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p, reach_, which, d_out);
     LF_HIP(hipGetLastError());
-    if (s && s != s_) {
-      hipEvent_t ev;
-      LF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      LF_HIP(hipEventRecord(ev, s_));
-      LF_HIP(hipStreamWaitEvent(s, ev, 0));
-      LF_HIP(hipEventDestroy(ev));
-    }
+    sync_to(s);
   }
 
  private:
@@ -3045,16 +3027,27 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipStreamSynchronize(s_));
     return *reinterpret_cast<int*>(host_small_);
   }
+  // Work queued on the consumer stream s (NULL = the null stream) after this
+  // call waits for everything queued so far on the engine's stream.  The
+  // engine's stream is non-blocking, so the null stream does not wait by
+  // itself.
   void sync_to(hipStream_t s) {
-    if (s && s != s_) {
-      hipEvent_t ev;
-      LF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      LF_HIP(hipEventRecord(ev, s_));
-      LF_HIP(hipStreamWaitEvent(s, ev, 0));
-      LF_HIP(hipEventDestroy(ev));
-    } else {
-      LF_HIP(hipStreamSynchronize(s_));
-    }
+    if (s == s_) return;
+    hipEvent_t ev;
+    LF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    LF_HIP(hipEventRecord(ev, s_));
+    LF_HIP(hipStreamWaitEvent(s, ev, 0));
+    LF_HIP(hipEventDestroy(ev));
+  }
+  // The engine's stream waits for everything queued so far on the producer
+  // stream s (NULL = the null stream) before it reads inputs written there.
+  void wait_for(hipStream_t s) {
+    if (s == s_) return;
+    hipEvent_t ev;
+    LF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    LF_HIP(hipEventRecord(ev, s));
+    LF_HIP(hipStreamWaitEvent(s_, ev, 0));
+    LF_HIP(hipEventDestroy(ev));
   }
   void ensure_pf_buffers(long long cap) {
     if (!pf_r_) {

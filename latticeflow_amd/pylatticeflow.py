@@ -24,6 +24,17 @@ from . import _lib
 from ._lib import check, lib
 
 
+def _torch_stream(device=None):
+    """Handle of torch's current HIP stream (0, the null stream, without torch)."""
+    try:
+        import torch
+    except ImportError:
+        return 0
+    if not torch.cuda.is_available():
+        return 0
+    return int(torch.cuda.current_stream(device).cuda_stream)
+
+
 def _names(fn):
     n = fn(None, 0)
     buf = ctypes.create_string_buffer(n + 1)
@@ -183,10 +194,15 @@ class LatticeFlow(object):
         check(lib().lf_add_energy_potentials(self._h, unary_a.ctypes.data, pairwise_a.ctypes.data))
 
     def addEnergyDevice(self, unary_ptr, pairwise_ptr, stream=None):
-        """Add energies already resident in device memory (int64, reference layout)."""
+        """Add energies already resident in device memory (int64, reference layout).
+
+        ``stream``: the HIP stream that wrote them (default: torch's current
+        stream); the library waits for it before reading."""
+        if stream is None:
+            stream = _torch_stream()
         check(lib().lf_add_energy_potentials_device(self._h, ctypes.c_void_p(unary_ptr),
                                                     ctypes.c_void_p(pairwise_ptr),
-                                                    ctypes.c_void_p(stream or 0)))
+                                                    ctypes.c_void_p(stream)))
 
     def resetEnergy(self):
         check(lib().lf_reset_energies(self._h))
@@ -233,8 +249,12 @@ class LatticeFlow(object):
         return out.reshape(*self._dimensions)
 
     def getCutDevice(self, out_ptr, which="smin", stream=None):
+        """Write the cut (int32, C order) into device memory; work queued on
+        ``stream`` (default: torch's current stream) afterwards sees it."""
         sel = {"smin": _lib.LF_CUT_SMIN, "smax": _lib.LF_CUT_SMAX}[which]
-        check(lib().lf_get_cut_device(self._h, ctypes.c_void_p(out_ptr), sel, ctypes.c_void_p(stream or 0)))
+        if stream is None:
+            stream = _torch_stream()
+        check(lib().lf_get_cut_device(self._h, ctypes.c_void_p(out_ptr), sel, ctypes.c_void_p(stream)))
 
     def flow(self):
         """Raw (doubled-unit) max-flow value of the last run."""
@@ -325,9 +345,13 @@ class LatticeFlowBatch(object):
         check(lib().lf_batch_add_energy_potentials(self._h, i, u.ctypes.data, p.ctypes.data))
 
     def addEnergyDevice(self, i, unary_ptr, pairwise_ptr, stream=None):
+        """Energies of lattice i from device memory; ``stream`` (default: torch's
+        current stream) is the stream that wrote them."""
+        if stream is None:
+            stream = _torch_stream()
         check(lib().lf_batch_add_energy_potentials_device(self._h, int(i), ctypes.c_void_p(unary_ptr),
                                                           ctypes.c_void_p(pairwise_ptr),
-                                                          ctypes.c_void_p(stream or 0)))
+                                                          ctypes.c_void_p(stream)))
 
     def resetEnergy(self):
         check(lib().lf_reset_energies(self._h))
@@ -427,7 +451,8 @@ def _tv_path_guided(X, lambdas, kernel, pred, delta, max_repairs=8):
     gs = _lib.lf_guided_stats()
     check(lib().lf_tv_path_guided(kernel.encode(), nx, ny, X.ctypes.data, lam.size, lam.ctypes.data,
                                   ctypes.c_void_p(pred.data_ptr()), float(delta), int(max_repairs),
-                                  R.ctypes.data, ctypes.byref(st), ctypes.byref(gs)))
+                                  R.ctypes.data, ctypes.byref(st), ctypes.byref(gs),
+                                  ctypes.c_void_p(_torch_stream(pred.device))))
     return R, st.as_dict(), gs.as_dict()
 
 

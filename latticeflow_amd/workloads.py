@@ -16,6 +16,7 @@ capacity -> energy mapping is E0 = c_s, E1 = c_t, E01 = E10 = w, E00 = E11 = 0.
                      thresholded into blobs plus N(0, 0.1),
                      w = round(300 exp(-diff^2 / 0.02) / |e|),
                      c_s = round(800 max(v - 0.5, 0)), c_t = round(800 max(0.5 - v, 0)).
+* ``noise26``        the same volume without the threshold (soft boundaries).
 * ``batch_sides``    side lengths of the batched workload (log-uniform 64..1024,
                      aspect ratio in [0.5, 2]).
 * ``batch_instances`` (shape, seed) of every lattice of the batched workload;
@@ -133,9 +134,22 @@ def value_noise(seed, shape, octaves=4, base=4):
 
 
 def blobs26(seed=3, shape=(256, 256, 256), kernel="Star3d_26"):
+    return _volume26(seed, shape, kernel, threshold=True)
+
+
+def noise26(seed=3, shape=(256, 256, 256), kernel="Star3d_26"):
+    """The volume of ``blobs26`` without the threshold into blobs: the value
+    noise itself plus N(0, 0.1), so that boundaries are soft and flow travels
+    (a harder 26-connected min-cut than the thresholded blobs)."""
+    return _volume26(seed, shape, kernel, threshold=False)
+
+
+def _volume26(seed, shape, kernel, threshold):
     rng = np.random.default_rng(seed + 1000003)
     offs, _ = edge_order(kernel, shape)
-    v = (value_noise(seed, shape) > 0.5).astype(np.float64)
+    v = value_noise(seed, shape)
+    if threshold:
+        v = (v > 0.5).astype(np.float64)
     v += rng.normal(0, 0.1, shape)
     cs = np.round(800 * np.maximum(v - 0.5, 0)).astype(np.int64)
     ct = np.round(800 * np.maximum(0.5 - v, 0)).astype(np.int64)

@@ -24,6 +24,27 @@
  *   orc_cuts           canonical outputs of a final state: the flow value,
  *                      S_min (residual reachability from excess) and S_max
  *                      (complement of "can reach a sink").
+ *   orc_pf_solve       ParametricFlowSolver::run (divide and conquer)
+ *                      solver/parametricflow/pf_solver.hpp:33-208 with
+ *                      setToMeanReductionPoint pf_flow_node.hpp:331-396
+ *   orc_tv_single      TVSolver::setup + runSingleLambda
+ *                      solver/tv/tv_solver.hpp:71-112,151-176
+ *   orc_tv_path        calculate2dTV(..., vector<double>) output,
+ *                      tv_solver.hpp:181-244,659-681
+ *
+ * How each function is pinned (tests/test_oracle*.py):
+ *   orc_reparam       the identity 2E(x) = K0 + sum red x + sum alpha |xi - xj|
+ *                     on random labellings; planted q12 / E01-E10 mistakes fail
+ *   orc_prflow_solve  flow = orc_dinic, S_min / S_max = exhaustive search on
+ *   orc_dinic         lattices of <= 16 nodes; the min energy from the flow
+ *   orc_cuts          equals the brute-force minimum
+ *   orc_pf_solve      Condat's exact 1-D TV (one quantum), the FP64 ROF dual on
+ *                     2-D lattices, exact conservation of sum r
+ *   orc_tv_single     the FP64 ROF dual through the lambda convention
+ *   orc_tv_path       the FP64 ROF dual through the path conventions
+ *                     (R - M = 2 s S (u - mean u), s = lambda / lambda_max) and
+ *                     path = scaled single-lambda solves; planted convention
+ *                     mistakes (halving, s = 1, lambda_max = last lambda) fail
  *
  * Network convention (the reference's doubled, "raw" units,
  * network_flow_node.hpp:234-240): per node a signed `red` (reduction: >0 is
@@ -100,7 +121,7 @@ static inline int rev_dir(int k, int i) { return (i + k / 2) % k; }
 int orc_reparam(int nd, const int64_t *shape, int k, const int32_t *offs,
                 const int64_t *unary, const int64_t *binary,
                 int64_t *red, int64_t *alpha) {
-  ulat_t L;
+  ulat_t L = {0};
   if (ulat_init(&L, nd, shape, k, offs)) return -1;
   const int kp = k / 2;
   memset(red, 0, sizeof(int64_t) * L.n);
@@ -136,7 +157,7 @@ int orc_reparam(int nd, const int64_t *shape, int k, const int32_t *offs,
 int orc_cuts(int nd, const int64_t *shape, int k, const int32_t *offs,
              const int64_t *red0, const int64_t *redF, const int64_t *res,
              int64_t *flow, uint8_t *smin, uint8_t *smax) {
-  ulat_t L;
+  ulat_t L = {0};
   if (ulat_init(&L, nd, shape, k, offs)) return -1;
   int64_t f = 0;
   for (int64_t v = 0; v < L.n; ++v) {
@@ -550,7 +571,7 @@ static int64_t padded_index(const ulat_t *L, const int64_t *pstride, int64_t beg
 int orc_prflow_solve(int nd, const int64_t *shape, int k, const int32_t *offs,
                      const int64_t *red_in, const int64_t *alpha_in,
                      int64_t *red_out, int64_t *res_out, uint8_t *flipped, int64_t *stats) {
-  ulat_t L;
+  ulat_t L = {0};
   if (ulat_init(&L, nd, shape, k, offs)) return -1;
   pr_t p;
   int64_t pstride[ORC_MAXD] = {0}, eb[ORC_MAXD] = {0};
@@ -591,7 +612,7 @@ int orc_prflow_solve(int nd, const int64_t *shape, int k, const int32_t *offs,
 int orc_dinic(int nd, const int64_t *shape, int k, const int32_t *offs,
               const int64_t *red, const int64_t *alpha,
               int64_t *flow, int64_t *red_out, int64_t *res_out) {
-  ulat_t L;
+  ulat_t L = {0};
   if (ulat_init(&L, nd, shape, k, offs)) return -1;
   const int64_t N = L.n, S = N, T = N + 1;
   int64_t *nbr = (int64_t *)malloc(sizeof(int64_t) * N * k);
@@ -824,7 +845,7 @@ static int64_t pf_run(pr_t *p, int64_t *off, int64_t *pts, int64_t npts, int64_t
 int orc_pf_solve(int nd, const int64_t *shape, int k, const int32_t *offs,
                  const int64_t *red_in, const int64_t *alpha_in,
                  int64_t *r_out, int64_t *level_out, int64_t *stats) {
-  ulat_t L;
+  ulat_t L = {0};
   if (ulat_init(&L, nd, shape, k, offs)) return -1;
   pr_t p;
   int64_t pstride[ORC_MAXD] = {0}, eb[ORC_MAXD] = {0};
@@ -897,7 +918,7 @@ int orc_tv_single(int64_t nx, int64_t ny, int k, const int32_t *offs, const doub
     red[i] += 2 * uf; /* addE1(0, uf) */
   }
   const double gsv = 0.5 / (S * max_lambda);
-  ulat_t L;
+  ulat_t L = {0};
   if (ulat_init(&L, 2, shape, k, offs)) { free(red); free(alpha); free(r); return -1; }
   for (int64_t v = 0; v < n; ++v)
     for (int e = 0; e < k / 2; ++e) {
@@ -951,7 +972,7 @@ int orc_tv_path(int64_t nx, int64_t ny, int k, const int32_t *offs, const double
   }
   if (lmax < 1e-8) lmax = 1e-8;
   int64_t shape[2] = {nx, ny};
-  ulat_t L;
+  ulat_t L = {0};
   if (ulat_init(&L, 2, shape, k, offs)) return -1;
   int64_t *red = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
   int64_t *alpha = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n * k));

@@ -44,8 +44,8 @@ def _check(oracle, kernel, shape, un, b, cap_bytes=8, tuning=None):
     return ref
 
 
-@pytest.mark.parametrize("kernel,nd", [("Full2d_4", 2), ("Full2d_8", 2), ("Star2d_16", 2),
-                                       ("Full3d_6", 3), ("Star3d_26", 3), ("Full1d_2", 1)])
+@pytest.mark.parametrize("kernel,nd", [("Full2d_4", 2), ("Full2d_8", 2), ("Full2d_12", 2), ("Star2d_16", 2),
+                                       ("Full3d_6", 3), ("Full3d_18", 3), ("Star3d_26", 3), ("Full1d_2", 1)])
 @pytest.mark.parametrize("cap_bytes", [4, 8])
 def test_random_small(oracle, kernel, nd, cap_bytes):
     rng = np.random.default_rng(hash((kernel, cap_bytes)) & 0xffff)
@@ -59,7 +59,7 @@ def test_random_small(oracle, kernel, nd, cap_bytes):
         offs, _ = oracle.edge_order(kernel, shape)
         un, b = _random_energies(rng, shape, offs)
         print("case", kernel, cap_bytes, shape, flush=True)
-        _check(oracle, kernel, shape, un, b, cap_bytes)
+        _check(oracle, kernel, shape, un, b, cap_bytes, tuning={"tile_shape": 1 + trial % 3})
 
 
 def test_grid4_256_config(oracle):

@@ -8,6 +8,10 @@
 * flows conserve sum r exactly;
 * the TV output equals M + S (u - mean u), u = ROF(f, 0.5 w_e / (S lambda)),
   checked against an independent FP64 dual solve (box-constrained, L-BFGS-B);
+* the regularization-path output (getRegularizationPath, tv_solver.hpp:181-244)
+  equals M + 2 s S (u - mean u) with s = lambda / lambda_max, against the same
+  ROF solves, and agrees with single-lambda solves the way the reference's own
+  path check compares them;
 * planted mistakes (floor instead of ceil threshold, a dropped pairwise weight)
   break these checks.
 """
@@ -77,6 +81,96 @@ def test_tv_single_matches_rof(oracle):
         u = oracle.rof_dual(f, f.shape, offs, 0.5 * w / (S * lam)).reshape(f.shape)
         Rx = M + S * (u - u.mean())
         assert np.abs(R - Rx).max() <= 1e-7 * np.abs(Rx).max()
+
+
+def _path_from_rof(oracle, f, lams, kernel):
+    """The regularization-path output of the reference's conventions, computed
+    from independent FP64 ROF solves.
+
+    The path network is built once at lambda_max (tv_solver.hpp:71-112, pairwise
+    0.5 w_e / (S lambda_max)); a point of the path at s = lambda / lambda_max
+    scales the function values by s (PFFlowNode::fv(scale) = multFVScale(base_fv,
+    scale), pf_flow_node.hpp:160-174,227-230), so by the homogeneity of the ROF
+    problem its raw value is s times the raw single-lambda value at lambda
+    (argmin 1/2|x - s y|^2 + c TV(x) = s argmin 1/2|x - y|^2 + (c/s) TV(x)).  The
+    traced raw values go through the solver's toFValue without the raw-to-energy
+    halving that runSingleLambda applies (tv_solver.hpp:219 vs :164), then are
+    re-centred and scaled by S again (:229-240).  With u = ROF(f, 0.5 w / (S
+    lambda)) in image units this gives R - M = 2 s S (u - mean u); lambda <= 0 is
+    the constant M (tv_solver.hpp:669-674: negative lambdas count as 0)."""
+    f = np.asarray(f, np.float64)
+    M = f.mean()
+    S = 2 * np.abs(f - M).max()
+    lam = np.maximum(np.asarray(lams, np.float64), 0.0)
+    lmax = max(lam.max(), 1e-8)
+    offs, w = oracle.edge_order(kernel, f.shape)
+    out = np.empty((lam.size,) + f.shape)
+    for l, lm in enumerate(lam):
+        if lm <= 0:
+            out[l] = M
+            continue
+        u = oracle.rof_dual(f, f.shape, offs, 0.5 * w / (S * lm)).reshape(f.shape)
+        out[l] = M + 2 * (lm / lmax) * S * (u - u.mean())
+    return out
+
+
+def _path_err(R, Rx, f):
+    """max deviation of the path from its pin, relative to the image's scale
+    S = 2 max|f - M| (a slice's spread is at most 2 s S)"""
+    f = np.asarray(f)
+    return np.abs(R - Rx).max() / (2 * np.abs(f - f.mean()).max())
+
+
+def test_tv_path_matches_rof(oracle):
+    """Every slice of the oracle's path restatement against independent ROF
+    solves mapped through the reference's path conventions."""
+    rng = np.random.default_rng(6)
+    for kernel, shape in (("Full2d_8", (6, 7)), ("Star2d_16", (5, 6)), ("Full2d_4", (7, 5))):
+        f = rng.random(shape)
+        # unsorted, with a repeat, a zero and a negative (counted as 0)
+        lams = [3.0, 12.0, 0.7, 6.0, 0.0, 3.0, -1.0]
+        R = oracle.tv_path(f, lams, kernel)
+        Rx = _path_from_rof(oracle, f, lams, kernel)
+        assert _path_err(R, Rx, f) <= 1e-7, (kernel, _path_err(R, Rx, f))
+
+
+def test_tv_path_consistent_with_single(oracle):
+    """The reference's own check of the path (scripts/test_reg_path.py:37-88,
+    solver/test_case2.cpp): each path slice against a single-lambda solve.
+    Under its conventions R_path - M = 2 (lambda / lambda_max) (R_single - M)."""
+    rng = np.random.default_rng(9)
+    f = rng.random((9, 8))
+    lams = [0.05, 0.9, 1.5, 2.5, 4.0]
+    R = oracle.tv_path(f, lams, "Full2d_8")
+    M = f.mean()
+    S = 2 * np.abs(f - M).max()
+    spread = 0.0
+    for l, lm in enumerate(lams):
+        Rs, _ = oracle.tv_single(f, lm, "Full2d_8")
+        want = M + 2 * (lm / max(lams)) * (Rs - M)
+        assert np.abs(R[l] - want).max() <= 1e-9 * S
+        spread = max(spread, np.abs(want - M).max())
+    assert spread > 0.1 * S  # not every slice is fully merged
+
+
+def test_tv_path_planted_mistakes_caught(oracle):
+    """Plausible mistakes in the path conventions, planted into the oracle's
+    output, fail the ROF pin: the raw-unit halving of the single-lambda output
+    applied to the path, s = 1 (no homogeneity scaling), lambda_max taken from
+    the last lambda of the list instead of the largest."""
+    rng = np.random.default_rng(7)
+    f = rng.random((6, 6))
+    lams = [4.0, 1.0, 2.0]
+    R = oracle.tv_path(f, lams, "Full2d_8")
+    Rx = _path_from_rof(oracle, f, lams, "Full2d_8")
+    assert _path_err(R, Rx, f) <= 1e-7
+    M = f.mean()
+    lam = np.asarray(lams)
+    halved = M + (R - M) / 2
+    no_scale = M + (R - M) / (lam / lam.max())[:, None, None]
+    last_max = M + (R - M) * (lam.max() / lam[-1])
+    for bad in (halved, no_scale, last_max):
+        assert _path_err(bad, Rx, f) > 1e-3
 
 
 def test_planted_mistakes_caught(oracle):
