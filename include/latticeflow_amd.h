@@ -228,6 +228,18 @@ int lf_tvnet_param_count(void);
 size_t lf_tvnet_work_bytes(int n, int64_t H, int64_t W);
 int lf_tvnet_forward(const float *params, int n, int64_t H, int64_t W, const float *f, const float *lam,
                      float *pred, void *work, size_t work_size, void *stream);
+/* The forward pass of lf_tvnet_forward without any allocation and without a
+ * workspace argument.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Its scratch (bf16 weights and
+ * activations) lives in a 64 MiB static device array that every GPU the
+ * library runs on holds until the process ends; n * H * W may be at most
+ * LF_TVNET_NOALLOC_MAX_PIXELS (larger inputs fail with LF_ERR_BAD_ARG).  All
+ * calls on one GPU share that array: calls that overlap on one GPU race.
+ * Output equals lf_tvnet_forward's. */
+#define LF_TVNET_NOALLOC_MAX_PIXELS 655360
+int lf_tvnet_forward_noalloc(const float *params, int n, int64_t H, int64_t W, const float *f, const float *lam,
+                             float *pred, void *stream);
 /* Gradient of the mean squared error against target: grad[param_count],
  * loss[1] (both overwritten). */
 int lf_tvnet_gradients(const float *params, int n, int64_t H, int64_t W, const float *f, const float *lam,

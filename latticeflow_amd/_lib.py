@@ -110,6 +110,7 @@ _SIG = {
     "lf_tvnet_param_count": (_I, []),
     "lf_tvnet_work_bytes": (ctypes.c_size_t, [_I, _I64, _I64]),
     "lf_tvnet_forward": (_I, [_P, _I, _I64, _I64, _P, _P, _P, _P, ctypes.c_size_t, _P]),
+    "lf_tvnet_forward_noalloc": (_I, [_P, _I, _I64, _I64, _P, _P, _P, _P]),
     "lf_tvnet_gradients": (_I, [_P, _I, _I64, _I64, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P]),
     "lf_tvnet_adam": (_I, [_P, _P, _P, _P, _I, ctypes.c_float, _P]),
     "lf_tv_single": (_I, [_CP, _I64, _I64, _P, ctypes.c_double, _P, ctypes.POINTER(lf_stats)]),

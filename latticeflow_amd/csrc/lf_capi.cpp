@@ -775,6 +775,14 @@ int lf_tvnet_forward(const float* params, int n, int64_t H, int64_t W, const flo
   });
 }
 
+int lf_tvnet_forward_noalloc(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam,
+                             float* pred, void* stream) {
+  return guarded([&] {
+    if (!params || !f || !lam || !pred) fail(LF_ERR_BAD_ARG, "null argument");
+    tvnet_forward_noalloc(params, n, H, W, f, lam, pred, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
 int lf_tvnet_gradients(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam,
                        const float* target, float* grad, float* loss, void* work, size_t work_size, void* stream) {
   return guarded([&] {

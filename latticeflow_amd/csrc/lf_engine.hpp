@@ -109,6 +109,9 @@ int tvnet_param_count();
 size_t tvnet_work_bytes(int n, int64_t H, int64_t W);
 void tvnet_forward(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam, float* pred,
                    void* work, size_t work_size, hipStream_t s);
+// The forward pass with its scratch in a static device array (no allocation).
+void tvnet_forward_noalloc(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam,
+                           float* pred, hipStream_t s);
 void tvnet_gradients(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam,
                      const float* target, float* grad, float* loss, void* work, size_t work_size, hipStream_t s);
 void tvnet_adam(float* params, const float* grad, float* m, float* v, int step, float lr, hipStream_t s);

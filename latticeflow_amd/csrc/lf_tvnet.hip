@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/latticeflow_amd.h"
 #include "lf_engine.hpp"
@@ -317,7 +318,9 @@ void check_shape(int n, int64_t H, int64_t W) {
   if (H > (1 << 20) || W > (1 << 20) || (long long)n * H > (1LL << 30)) fail(LF_ERR_BAD_ARG, "batch too large");
 }
 
-void forward(const float* params, int n, int H, int W, const float* f, const float* lam, Work w, hipStream_t s) {
+// Forward pass into `out` (P floats); uses w.wq, w.x1, w.y1, w.y2.
+void forward_to(const float* params, int n, int H, int W, const float* f, const float* lam, const Work& w,
+                float* out, hipStream_t s) {
   const long long P = (long long)n * H * W;
   const unsigned rows = (unsigned)((long long)n * H * ((W + 63) / 64));
   hipLaunchKernelGGL(k_tvnet_wq, dim3(nblk(NPARAM)), dim3(CT), 0, s, params, w.wq);
@@ -327,7 +330,45 @@ void forward(const float* params, int n, int H, int W, const float* f, const flo
   hipLaunchKernelGGL((k_tvnet_fwd<true, false>), dim3(rows), dim3(CT), 0, s, w.y1, n, H, W, w.wq + LAYER,
                      params + LAYER + WSZ, w.y2, (float*)nullptr, (const float*)nullptr);
   hipLaunchKernelGGL((k_tvnet_fwd<false, true>), dim3(rows), dim3(CT), 0, s, w.y2, n, H, W, w.wq + 2 * LAYER,
-                     params + 2 * LAYER + WSZ, (unsigned short*)nullptr, w.pred, f);
+                     params + 2 * LAYER + WSZ, (unsigned short*)nullptr, out, f);
+}
+
+void forward(const float* params, int n, int H, int W, const float* f, const float* lam, Work w, hipStream_t s) {
+  forward_to(params, n, H, W, f, lam, w, w.pred, s);
+}
+
+// ---- forward pass without allocation ---------------------------------------------
+// Scratch of tvnet_forward_noalloc: the bf16 weights and the three activation
+// planes of one call live in this file-scope device array; every GPU the
+// library runs on holds its own copy until the process ends.  Host code never
+// passes the array's name to a kernel: it takes the device address of the
+// current GPU's copy from hipGetSymbolAddress (once per device) and uses that.
+// Calls that overlap on one GPU share the array and race.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static_assert((size_t)LF_TVNET_NOALLOC_MAX_PIXELS * 3 * C * sizeof(unsigned short) +
+                      ((NPARAM * 2 + 255) / 256) * 256 + 3 * 256 <= kNoAllocBytes,
+              "the stated pixel limit must fit the scratch array");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current GPU's scratch (looked up on the first call on
+// each device, with that device current).
+void* scratch_address() {
+  int dev = 0;
+  LF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(LF_ERR_DEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      fail(LF_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
 }
 
 void gradients(const float* params, int n, int H, int W, const float* f, const float* lam, const float* target,
@@ -372,6 +413,25 @@ void tvnet_forward(const float* params, int n, int64_t H, int64_t W, const float
   Work w = carve(work, P);
   forward(params, n, (int)H, (int)W, f, lam, w, s);
   LF_HIP(hipMemcpyAsync(pred, w.pred, sizeof(float) * P, hipMemcpyDeviceToDevice, s));
+  LF_HIP(hipGetLastError());
+}
+
+void tvnet_forward_noalloc(const float* params, int n, int64_t H, int64_t W, const float* f, const float* lam,
+                           float* pred, hipStream_t s) {
+  check_shape(n, H, W);
+  const long long P = (long long)n * H * W;
+  if (P > LF_TVNET_NOALLOC_MAX_PIXELS)
+    fail(LF_ERR_BAD_ARG, "n * H * W = " + std::to_string(P) + " pixels exceeds the allocation-free limit of " +
+                             std::to_string(LF_TVNET_NOALLOC_MAX_PIXELS) + " (LF_TVNET_NOALLOC_MAX_PIXELS)");
+  char* c = reinterpret_cast<char*>(scratch_address());
+  const size_t act = ((size_t)P * C * sizeof(unsigned short) + 255) / 256 * 256;
+  Work w{};
+  w.wq = reinterpret_cast<unsigned short*>(c);
+  c += ((NPARAM * 2 + 255) / 256) * 256;
+  w.x1 = reinterpret_cast<unsigned short*>(c);
+  w.y1 = reinterpret_cast<unsigned short*>(c + act);
+  w.y2 = reinterpret_cast<unsigned short*>(c + 2 * act);
+  forward_to(params, n, (int)H, (int)W, f, lam, w, pred, s);
   LF_HIP(hipGetLastError());
 }
 
