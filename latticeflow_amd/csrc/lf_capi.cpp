@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/latticeflow_amd.h"
+#include "lf_bk.hpp"
 #include "lf_engine.hpp"
 #include "lf_kernels.hpp"
 #include "lf_tv.hpp"
@@ -28,7 +29,10 @@ namespace {
 
 thread_local std::string g_last_error;
 
-const char* const kSolvers[] = {"LatticeEnergyMinimizer", "LatticeLevelReductions"};
+// LatticeEnergyMinimizer / LatticeLevelReductions run on the device; the
+// reference's optional BKGCEnergyMinimizer (wscript:21-23,187-190) is the CPU
+// cross-check solver of lf_bk.cpp, used only when a caller names it.
+const char* const kSolvers[] = {"LatticeEnergyMinimizer", "LatticeLevelReductions", "BKGCEnergyMinimizer"};
 
 bool valid_solver(const std::string& s) {
   for (const char* n : kSolvers)
@@ -81,6 +85,7 @@ struct lf_handle {
   std::vector<double> geocut; // k
   LatticeGeom geom;
   std::unique_ptr<MinCutEngine> eng;
+  std::unique_ptr<BkSolver> bk;  // BKGCEnergyMinimizer (CPU); eng is then null
   bool levels = false;  // LatticeLevelReductions (parametric divide and conquer)
   bool batch = false;   // lattices packed on one canvas
   std::vector<BatchRect> rects;
@@ -178,6 +183,11 @@ int lf_create(lf_handle** out, const char* solver, const char* kernel, int ndim,
       g.dy.push_back(ndim == 3 ? o[1] : (ndim == 2 ? o[0] : 0));
       g.dx.push_back(o[ndim - 1]);
     }
+    if (h->solver == "BKGCEnergyMinimizer") {
+      h->bk.reset(new BkSolver(ndim, h->shape, h->k, h->offs));
+      *out = h.release();
+      return;
+    }
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
       fail(LF_ERR_DEVICE, "no HIP device available");
@@ -214,6 +224,16 @@ int lf_add_energy_potentials_device(lf_handle* h, const int64_t* du, const int64
   return guarded([&] {
     if (!h || !du || !db) fail(LF_ERR_BAD_ARG, "null argument");
     if (h->batch) fail(LF_ERR_STATE, "a batch handle takes energies per lattice (lf_batch_add_energy_potentials)");
+    if (h->bk) {
+      // the CPU solver reads host copies
+      std::vector<int64_t> u((size_t)h->n * 2), b((size_t)h->n * (h->k / 2) * 4);
+      hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+      LF_HIP(hipMemcpyAsync(u.data(), du, u.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      LF_HIP(hipMemcpyAsync(b.data(), db, b.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      LF_HIP(hipStreamSynchronize(s));
+      h->bk->set_energies(u.data(), b.data());
+      return;
+    }
     h->eng->add_energies_device(du, db, reinterpret_cast<hipStream_t>(stream));
   });
 }
@@ -222,6 +242,10 @@ int lf_add_energy_potentials(lf_handle* h, const int64_t* unary, const int64_t* 
   return guarded([&] {
     if (!h || !unary || !binary) fail(LF_ERR_BAD_ARG, "null argument");
     if (h->batch) fail(LF_ERR_STATE, "a batch handle takes energies per lattice (lf_batch_add_energy_potentials)");
+    if (h->bk) {
+      h->bk->set_energies(unary, binary);
+      return;
+    }
     const size_t nu = (size_t)h->n * 2, nb = (size_t)h->n * (h->k / 2) * 4;
     int64_t *du = nullptr, *db = nullptr;
     hipStream_t s = h->eng->stream();
@@ -245,6 +269,11 @@ int lf_add_energy_potentials(lf_handle* h, const int64_t* unary, const int64_t* 
 int lf_reset_energies(lf_handle* h) {
   return guarded([&] {
     if (!h) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->bk) {
+      std::vector<int64_t> u((size_t)h->n * 2, 0), b((size_t)h->n * (h->k / 2) * 4, 0);
+      h->bk->set_energies(u.data(), b.data());
+      return;
+    }
     h->eng->reset_energies();
   });
 }
@@ -266,6 +295,10 @@ int lf_get_tuning(const lf_handle* h, lf_tuning* t) {
 int lf_autotune(lf_handle* h, int reps, lf_tuning* best) {
   return guarded([&] {
     if (!h) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->bk) {  // the CPU solver has nothing to tune
+      if (best) *best = h->tuning;
+      return;
+    }
     if (reps < 1) reps = 1;
     // one untimed warm-up, then the mean of `reps` timed solves (the
     // schedule of a push-relabel run is not deterministic, so a candidate
@@ -316,7 +349,8 @@ int lf_run(lf_handle* h) {
   return guarded([&] {
     if (!h) fail(LF_ERR_BAD_ARG, "null argument");
     const auto t0 = std::chrono::steady_clock::now();
-    if (h->levels) h->eng->pf_run(h->tuning);
+    if (h->bk) h->bk->run();
+    else if (h->levels) h->eng->pf_run(h->tuning);
     else h->eng->run(h->tuning);
     h->run_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   });
@@ -332,6 +366,12 @@ int lf_run_time(const lf_handle* h, double* seconds) {
 int lf_get_stats(const lf_handle* h, lf_stats* st) {
   return guarded([&] {
     if (!h || !st) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->bk) {
+      *st = lf_stats{};
+      st->cycles = h->bk->augmentations();
+      st->seconds = h->run_time;
+      return;
+    }
     *st = h->eng->stats;
   });
 }
@@ -339,6 +379,15 @@ int lf_get_stats(const lf_handle* h, lf_stats* st) {
 int lf_get_cut_device(const lf_handle* h, int32_t* d_out, int which, void* stream) {
   return guarded([&] {
     if (!h || !d_out) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->bk) {
+      if (which != LF_CUT_SMIN && which != LF_CUT_SMAX) fail(LF_ERR_BAD_ARG, "bad cut selector");
+      std::vector<int32_t> c((size_t)h->n);
+      h->bk->cut(c.data(), which);
+      hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+      LF_HIP(hipMemcpyAsync(d_out, c.data(), c.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      LF_HIP(hipStreamSynchronize(s));
+      return;
+    }
     if (h->levels) {
       if (!h->eng->pf_solved) fail(LF_ERR_STATE, "run() has not been called");
       h->eng->pf_cut_device(d_out, reinterpret_cast<hipStream_t>(stream));
@@ -354,6 +403,11 @@ int lf_get_cut_device(const lf_handle* h, int32_t* d_out, int which, void* strea
 int lf_get_cut(const lf_handle* h, int32_t* out, int which) {
   return guarded([&] {
     if (!h || !out) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->bk) {
+      if (which != LF_CUT_SMIN && which != LF_CUT_SMAX) fail(LF_ERR_BAD_ARG, "bad cut selector");
+      h->bk->cut(out, which);
+      return;
+    }
     hipStream_t s = h->eng->stream();
     int32_t* d = nullptr;
     LF_HIP(hipMallocAsync((void**)&d, sizeof(int32_t) * h->n, s));
@@ -382,7 +436,7 @@ int lf_get_cut(const lf_handle* h, int32_t* out, int which) {
 int lf_get_levels(const lf_handle* h, int64_t* r_out) {
   return guarded([&] {
     if (!h || !r_out) fail(LF_ERR_BAD_ARG, "null argument");
-    if (!h->levels) fail(LF_ERR_STATE, "level values exist only for LatticeLevelReductions");
+    if (h->bk || !h->levels) fail(LF_ERR_STATE, "level values exist only for LatticeLevelReductions");
     if (!h->eng->pf_solved) fail(LF_ERR_STATE, "run() has not been called");
     hipStream_t s = h->eng->stream();
     int64_t* d = nullptr;
@@ -404,6 +458,11 @@ int lf_get_levels(const lf_handle* h, int64_t* r_out) {
 int lf_get_flow(const lf_handle* h, int64_t* flow) {
   return guarded([&] {
     if (!h || !flow) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->bk) {
+      if (!h->bk->solved()) fail(LF_ERR_STATE, "run() has not been called");
+      *flow = h->bk->flow();
+      return;
+    }
     if (!h->eng->solved) fail(LF_ERR_STATE, "run() has not been called");
     *flow = h->eng->flow;
   });

@@ -402,6 +402,91 @@ class LatticeFlowBatch(object):
         return [flat[self._offsets[i]:self._offsets[i + 1]].reshape(s) for i, s in enumerate(self._shapes)]
 
 
+def _benchmark_energies(X, kernel, edges):
+    """Energies of the reference's image benchmark (pyx:305-398), vectorised.
+
+    2-D kernels: one node per pixel with the pixel's RGB colour; 3-D kernels:
+    nodes (x, y, z) over an H x W x W volume with colour (X[x, y] + X[x, z]) / 2.
+    E0 = 50000 on pure black nodes, E1 = 50000 on pure white ones; every
+    in-lattice edge e gets E01 = E10 = exp(-|c1 - c2|^2) / |delta_e|^2, and the
+    smallest E01 of the whole array (0 wherever an edge leaves the lattice) is
+    subtracted from the E01 entries (pyx:393-394)."""
+    X = np.asarray(X, dtype=np.float64)[:, :, :3]
+    dim = getKernelDimension(kernel)
+    H, W = X.shape[0], X.shape[1]
+    if dim == 2:
+        shape = (H, W)
+        col = X
+    elif dim == 3:
+        shape = (H, W, W)
+        col = (X[:, :, None, :] + X[:, None, :, :]) / 2
+    else:
+        raise ValueError("Benchmark for dimension %d not implemented (kernel=%s)" % (dim, kernel))
+    n = int(np.prod(shape))
+    flat = col.reshape(n, 3)
+    unary = np.zeros((n, 2))
+    black = (flat[:, 0] == flat[:, 1]) & (flat[:, 1] == flat[:, 2]) & (flat[:, 2] == 0)
+    white = (flat[:, 0] == flat[:, 1]) & (flat[:, 1] == flat[:, 2]) & (flat[:, 2] == 1)
+    unary[black, 0] = 50000
+    unary[white, 1] = 50000
+    binary = np.zeros((n, len(edges), 4))
+    for ei, dlt in enumerate(np.asarray(edges)):
+        scale = 1.0 / float(np.sum(dlt.astype(np.float64) ** 2))
+        src = tuple(slice(max(0, -int(d)), shape[a] - max(0, int(d))) for a, d in enumerate(dlt))
+        dst = tuple(slice(max(0, int(d)), shape[a] + min(0, int(d))) for a, d in enumerate(dlt))
+        c1 = col[src]
+        c2 = col[dst]
+        d = np.exp(-np.sum((c1 - c2) ** 2, axis=-1)) * scale
+        plane = np.zeros(shape)
+        plane[src] = d
+        binary[:, ei, 1] = plane.reshape(-1)
+        binary[:, ei, 2] = plane.reshape(-1)
+    binary[:, :, 1:2] -= binary[:, :, 1].min()
+    return shape, unary, binary
+
+
+def _runBenchmark(image_file, kernel, solver, check_result=False):
+    """pyx:305-435: segment an RGB image (a PNG path, read with matplotlib's
+    imread as the reference does, or an array H x W x 3+) with ``solver``, print
+    the result and return runTime().  With ``check_result`` the same energies
+    are solved by the CPU solver "BKGCEnergyMinimizer" and the cuts compared.
+    The printed solution hash is md5 of the cut's bytes (the reference hashes a
+    Python 2 pickle of it)."""
+    import hashlib
+    if isinstance(image_file, str):
+        from matplotlib.pyplot import imread
+        Xo = imread(image_file)
+        if not Xo.size:
+            raise IOError("Error loading image %s." % image_file)
+    else:
+        Xo = np.asarray(image_file)
+    if Xo.ndim != 3:
+        raise IOError("Error loading image %s; need RGB images." % (image_file if isinstance(image_file, str) else "array"))
+    lf = LatticeFlow(kernel, solver, (Xo.shape[0], Xo.shape[1]) if getKernelDimension(kernel) == 2
+                     else (Xo.shape[0], Xo.shape[1], Xo.shape[1]))
+    print("Calculating potentials.")
+    shape, unary, binary = _benchmark_energies(Xo, kernel, lf.edges)
+    lf.addEnergy(unary, binary)
+    lf.run()
+    cut = lf.getCut().ravel()
+    runtime = lf.runTime()
+    print("Final result: %d / %d turned on, solution hash = %s, ran in %f seconds."
+          % (cut.sum(), cut.shape[0], hashlib.md5(cut.tobytes()).hexdigest()[:8], runtime))
+    if check_result:
+        print("Checking solution:")
+        lf2 = LatticeFlow(kernel, "BKGCEnergyMinimizer", shape)
+        lf2.addEnergy(unary, binary)
+        lf2.run()
+        print("BKGC Solver ran in %f seconds." % lf2.runTime())
+        cut2 = lf2.getCut().ravel()
+        if (cut != cut2).sum() != 0:
+            print("Solution different: %d falsely 0, %d falsely 1"
+                  % (((cut == 0) & (cut2 == 1)).sum(), ((cut == 1) & (cut2 == 0)).sum()))
+        else:
+            print("Solutions exactly match.")
+    return runtime
+
+
 def _tv_single(X, lam, kernel):
     X = np.ascontiguousarray(X, dtype=np.float64)
     if X.ndim != 2:

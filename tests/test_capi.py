@@ -41,7 +41,7 @@ def test_names_and_dimensions(oracle):
     for n, (nd, _) in fam.items():
         assert L.getKernelDimension(n) == nd
     assert L.getKernelDimension("Full3d_26") == 3
-    assert set(L.validSolverNames()) == {"LatticeEnergyMinimizer", "LatticeLevelReductions"}
+    assert set(L.validSolverNames()) == {"LatticeEnergyMinimizer", "LatticeLevelReductions", "BKGCEnergyMinimizer"}
     with pytest.raises(ValueError):
         L.getKernelDimension("Full2d_5")
     with pytest.raises(ValueError):

@@ -145,3 +145,38 @@ def test_level_reductions_device_cut(oracle):
     ref = oracle.pf_solve(shape, lf.directions(), red, alpha)
     assert r.sum() == ref["r"].sum()
     assert np.abs(r - ref["r"]).max() <= 2
+
+
+def test_run_benchmark_device_vs_bk(tmp_path, capsys):
+    """The reference's image benchmark (pyx:305-435) on the device solver with
+    check_result: the CPU BK solver must give exactly the same cut (2-D and the
+    benchmark's 3-D volume form)."""
+    from matplotlib.image import imsave
+    from latticeflow_amd.pylatticeflow import _runBenchmark
+    rng = np.random.default_rng(2)
+    for kernel, (H, W) in (("Full2d_8", (120, 160)), ("Star2d_16", (64, 90)), ("Full3d_6", (24, 20))):
+        X = rng.random((H, W, 3))
+        X[: H // 3, : W // 2] = 0.0
+        X[H // 2:, W // 2:] = 1.0
+        path = str(tmp_path / ("img_%s.png" % kernel))
+        imsave(path, X)
+        print("case", kernel, H, W, flush=True)
+        _runBenchmark(path, kernel, "LatticeEnergyMinimizer", check_result=True)
+        out = capsys.readouterr().out
+        assert "Solutions exactly match." in out, out
+
+
+def test_bk_equals_device_on_segmentation(oracle):
+    """BKGCEnergyMinimizer (CPU) and the device solver on the same 512x384
+    segmentation lattice: same flow, S_min and S_max."""
+    from latticeflow_amd import LatticeFlow, workloads as W
+    w = W.seg8(seed=12, shape=(512, 384))
+    out = {}
+    for solver in ("LatticeEnergyMinimizer", "BKGCEnergyMinimizer"):
+        lf = LatticeFlow(w["kernel"], solver, w["shape"], numerical_precision=1.0)
+        lf.addEnergyInt(w["unary"], w["pairwise"])
+        lf.run()
+        out[solver] = (lf.flow(), lf.getCut("smin"), lf.getCut("smax"))
+    a, b = out["LatticeEnergyMinimizer"], out["BKGCEnergyMinimizer"]
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
