@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblatticeflow_amd.so")
-SOURCES = ["lf_mincut.hip", "lf_mincut_wide.hip", "lf_mincut_wide32.hip", "lf_mincut_wide48.hip", "lf_tvnet.hip",
+SOURCES = ["lf_mincut.hip", "lf_mincut_generic.hip", "lf_mincut_wide.hip", "lf_mincut_wide32.hip", "lf_mincut_wide48.hip", "lf_tvnet.hip",
            "lf_tv.cpp", "lf_bk.cpp", "lf_capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -174,6 +174,8 @@ int lf_create(lf_handle** out, const char* solver, const char* kernel, int ndim,
     LatticeGeom& g = h->geom;
     g.ndim = ndim;
     g.k = h->k;
+    g.dims = h->shape;
+    g.offsets = h->offs;
     g.D = ndim == 3 ? shape[0] : 1;
     g.H = ndim == 3 ? shape[1] : (ndim == 2 ? shape[0] : 1);
     g.W = shape[ndim - 1];

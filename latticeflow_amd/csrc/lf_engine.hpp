@@ -35,6 +35,10 @@ struct LatticeGeom {
   int64_t D = 1, H = 1, W = 1;
   int k = 0;
   std::vector<int> dz, dy, dx;
+  // the lattice's own shape and directions (k * ndim, lattice order): the
+  // node-parallel engine of lattices with more than three dimensions uses these
+  std::vector<int64_t> dims;
+  std::vector<int> offsets;
   int64_t n() const { return D * H * W; }
 };
 
@@ -95,6 +99,9 @@ class MinCutEngine {
 };
 
 std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_bytes);
+// Node-parallel engine for lattices of up to four dimensions (lf_mincut_generic.hip):
+// the 4-D kernels.
+std::unique_ptr<MinCutEngine> make_generic_engine(const LatticeGeom& g, int cap_bytes);
 // Engines of the wider 1-D / 2-D neighbourhoods (radius 2 to 4) that the main
 // factory does not build itself (lf_mincut_wide.hip); nullptr when none fits.
 std::unique_ptr<MinCutEngine> make_wide_engine(const LatticeGeom& g, int cap_bytes);

@@ -3,7 +3,8 @@
 // Restates, in C++, what the reference generates at configure time:
 //   * the Full / Star offset sets and their names "%(type)s%(dim)dd_%(n)d"
 //     (install/code_setup.py:40, :199-256), over the radii of
-//     install/code_setup.py:75-78 (2D up to radius 4, 3D up to radius 2);
+//     install/code_setup.py:75-78 (2D up to radius 4, 3D up to radius 2, and
+//     the 4D kernels of radius 1 that --4d adds, :141-153, wscript:64-67);
 //   * the Cauchy-Crofton geocut weights of 2D kernels (install/code_setup.py:163-197);
 //   * the lattice edge order: directions sorted by their linear jump over the
 //     padded strides, positive half ascending, entry i + k/2 = -entry i
@@ -137,8 +138,10 @@ inline const std::map<std::string, KernelDef>& kernel_registry() {
       m[k.name] = k;
     };
     for (int r = 1; r <= 4; ++r) add("Full", 1, detail::full_edges(1, r));
-    for (int dim = 2; dim <= 3; ++dim) {
-      int maxr = dim == 2 ? 4 : 2;
+    // 2D up to radius 4, 3D up to radius 2, 4D radius 1 (the --4d default,
+    // wscript:64-67)
+    for (int dim = 2; dim <= 4; ++dim) {
+      int maxr = dim == 2 ? 4 : (dim == 3 ? 2 : 1);
       for (double r : detail::radius_intervals(dim, maxr)) {
         add("Full", dim, detail::full_edges(dim, r));
         add("Star", dim, detail::star_edges(dim, r));

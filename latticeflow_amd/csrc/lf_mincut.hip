@@ -3413,6 +3413,7 @@ int max_abs(const std::vector<int>& v) {
 #ifndef LF_MINCUT_WIDE_TU
 std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_bytes) {
   if (cap_bytes != 4 && cap_bytes != 8) fail(LF_ERR_BAD_ARG, "cap_bytes must be 4 or 8");
+  if (g.ndim == 4) return make_generic_engine(g, cap_bytes);
   const int rz = max_abs(g.dz), ry = max_abs(g.dy), rx = max_abs(g.dx);
   const bool is3d = g.ndim == 3;
   if (!is3d && rz == 0 && ry <= 1 && rx <= 1) {

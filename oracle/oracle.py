@@ -89,8 +89,9 @@ def geocut_weights(edges):
 
 def kernel_family():
     """All kernels the reference generates by default (code_setup.py:12,71-155):
-    2D Full/Star up to radius 4, 3D Full/Star up to radius 2, plus the 1D ones
-    this build also registers (Full1d_2 .. Full1d_8, radius 1..4).
+    2D Full/Star up to radius 4, 3D Full/Star up to radius 2, the 4D Full/Star
+    of radius 1 that its --4d option adds (max-4d-radius 1, wscript:64-67), plus
+    the 1D ones this build also registers (Full1d_2 .. Full1d_8, radius 1..4).
 
     Returns {name: (ndim, offsets list in generator order)}.
     """
@@ -98,7 +99,7 @@ def kernel_family():
     for r in range(1, 5):
         e = _full(1, r)
         out["Full1d_%d" % len(e)] = (1, e)
-    for dim, maxr in ((2, 4), (3, 2)):
+    for dim, maxr in ((2, 4), (3, 2), (4, 1)):
         for r in _radius_intervals(dim, maxr):
             for typ, fn in (("Full", _full), ("Star", _star)):
                 e = fn(dim, r)

@@ -103,3 +103,17 @@ def test_bk_rejects_nonsubmodular():
     lf.addEnergyInt(un, b)
     with pytest.raises(ValueError):
         lf.run()
+
+
+def test_bk_4d(oracle):
+    rng = np.random.default_rng(4)
+    for _ in range(4):
+        shape = tuple(int(x) for x in rng.integers(1, 6, 4))
+        offs, _ = oracle.edge_order("Full4d_8", shape)
+        un, b = _energies(rng, shape, offs)
+        lf = _bk("Full4d_8", shape, un, b)
+        red, alpha = oracle.reparam(shape, lf.directions(), un, b)
+        ref = oracle.prflow(shape, lf.directions(), red, alpha)
+        assert lf.flow() == ref["flow"]
+        assert np.array_equal(lf.getCut("smin").reshape(-1), ref["smin"].astype(np.int32))
+        assert np.array_equal(lf.getCut("smax").reshape(-1), ref["smax"].astype(np.int32))

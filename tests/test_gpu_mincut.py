@@ -241,3 +241,20 @@ def test_bfs_list_overflow(oracle):
     assert np.array_equal(red > 0, sink) and np.all(red[~sink] < 0)
     for tun in ({}, {"relabel_launches": 1, "tile_shape": 2}):
         _check(oracle, "Full2d_8", shape, un, b, cap_bytes=4, tuning=tun)
+
+
+@pytest.mark.parametrize("kernel", ["Full4d_8", "Star4d_8"])
+@pytest.mark.parametrize("cap_bytes", [4, 8])
+def test_4d_lattices(oracle, kernel, cap_bytes):
+    """4-D kernels (install/code_setup.py:141-153, the reference's --4d
+    build) on the node-parallel device engine: flow and both canonical cuts
+    bit-exact, including degenerate extents of 1."""
+    rng = np.random.default_rng(hash((kernel, cap_bytes, 4)) & 0xffff)
+    for trial in range(5):
+        shape = tuple(int(x) for x in rng.integers(1, 9, 4))
+        if trial == 0:
+            shape = (1, 5, 1, 6)
+        offs, _ = oracle.edge_order(kernel, shape)
+        un, b = _random_energies(rng, shape, offs, cap=40, wcap=15)
+        print("case", kernel, cap_bytes, shape, flush=True)
+        _check(oracle, kernel, shape, un, b, cap_bytes)
