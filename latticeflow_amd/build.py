@@ -64,5 +64,32 @@ def build(verbose: bool = False, jobs: int = 8, incremental: bool = False) -> st
     return LIB
 
 
+def build_ext(verbose: bool = False) -> str:
+    """The Cython extension ext/pylatticeflow (over include/latticeflow_amd/*.hpp),
+    linked against the library built above with a relative run path."""
+    import sysconfig
+    from Cython.Build import cythonize  # noqa: F401  (Cython 3 is the translator)
+    from Cython.Compiler import Main as cymain
+    ext = os.path.join(HERE, "ext")
+    pyx = os.path.join(ext, "pylatticeflow.pyx")
+    cpp = os.path.join(HERE, "_build", "pylatticeflow.cpp")
+    os.makedirs(os.path.dirname(cpp), exist_ok=True)
+    opts = cymain.CompilationOptions(cplus=True, language_level=3, output_file=cpp)
+    res = cymain.compile(pyx, opts)
+    if res.num_errors:
+        raise RuntimeError("cython failed for pylatticeflow.pyx")
+    out = os.path.join(ext, "pylatticeflow" + sysconfig.get_config_var("EXT_SUFFIX"))
+    inc = [sysconfig.get_paths()["include"], os.path.join(HERE, "..", "include")]
+    cmd = ["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-w"] + ["-I" + i for i in inc] + [
+        cpp, "-o", out + ".tmp", "-L" + HERE, "-llatticeflow_amd", "-Wl,-rpath,$ORIGIN/..",
+        "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))
+    print(build_ext(verbose="-v" in sys.argv))

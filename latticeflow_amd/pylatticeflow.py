@@ -58,6 +58,71 @@ def getKernelDimension(kernel: str) -> int:
     return lib().lf_kernel_dimension(kernel.encode())
 
 
+def _energy_arrays(dimensions, n_nodes, n_filling_edges, unary_energy, pairwise_energy):
+    """pyx:157-267: validate integral energies of a lattice of ``dimensions``
+    and return them as contiguous int64 arrays (n, 2) and (n, k/2, 4), raising
+    the reference's ValueError messages on malformed shapes."""
+    unary_energy_error = (
+        "Unary energy must be either a 2d array with first dimension "
+        "matching the number of nodes and second dimension containing E0 and E1, "
+        "or (#dimensions + 1) with the first matching the lattice coordinates and "
+        "the last containing E0 and E1.")
+    if unary_energy.ndim == 2:
+        if unary_energy.shape[0] != n_nodes:
+            raise ValueError(unary_energy_error + (" (# nodes = %d; # unary_array_nodes = %d)"
+                                                   % (n_nodes, unary_energy.shape[0])))
+        if unary_energy.shape[1] != 2:
+            raise ValueError(unary_energy_error + (" (Second dimension has shape %d; 2 required.)"
+                                                   % unary_energy.shape[1]))
+        unary_a = unary_energy
+    elif unary_energy.ndim == len(dimensions) + 1:
+        for i in range(len(dimensions)):
+            if unary_energy.shape[i] != dimensions[i]:
+                raise ValueError(unary_energy_error + (" (dimension %d = %d; unary_array shape is %d)"
+                                                       % (i, dimensions[i], unary_energy.shape[i])))
+        if unary_energy.shape[-1] != 2:
+            raise ValueError(unary_energy_error + (" (Last dimension has shape %d; 2 required.)"
+                                                   % unary_energy.shape[-1]))
+        unary_a = unary_energy.reshape(-1, 2)
+    else:
+        raise ValueError(unary_energy_error + " (wrong number of dimensions)")
+
+    pairwise_energy_error = (
+        "Pairwise energy must be either: (1) a 3d array with first dimension "
+        "matching the number of nodes, second dimension containing the edge direction output, "
+        "and third dimension containing E00, E01, E10, and E11 (with E01 being energy of "
+        "source node off and dest node on.), or (2) a (#dimensions "
+        "+ 2) array with last two dimensions being the same and first dimensions corresponding "
+        "to the lattice coordinates. ")
+    bad_edge_error = (pairwise_energy_error +
+                      " (wrong number of edges specified in second-to-last dimension; got %d, needed %d).")
+    bad_last_error = pairwise_energy_error + " (wrong number of pairwise energy terms; needed 4, got %d."
+    ne = n_filling_edges
+    if pairwise_energy.ndim == 3:
+        if pairwise_energy.shape[0] != n_nodes:
+            raise ValueError(pairwise_energy_error + (" (# nodes = %d; # nodes in pairwise_energy = %d)"
+                                                      % (n_nodes, pairwise_energy.shape[0])))
+        if pairwise_energy.shape[1] != ne:
+            raise ValueError(bad_edge_error % (pairwise_energy.shape[1], ne))
+        if pairwise_energy.shape[2] != 4:
+            raise ValueError(bad_last_error % pairwise_energy.shape[2])
+        pairwise_a = pairwise_energy
+    elif pairwise_energy.ndim == len(dimensions) + 2:
+        for i in range(len(dimensions)):
+            if pairwise_energy.shape[i] != dimensions[i]:
+                raise ValueError(pairwise_energy_error + (" (dimension %d shape is %d; pairwise_array shape is %d)"
+                                                          % (i, dimensions[i], pairwise_energy.shape[i])))
+        if pairwise_energy.shape[-2] != ne:
+            raise ValueError(bad_edge_error % (pairwise_energy.shape[-2], ne))
+        if pairwise_energy.shape[-1] != 4:
+            raise ValueError(bad_last_error % pairwise_energy.shape[-1])
+        pairwise_a = pairwise_energy.reshape(-1, ne, 4)
+    else:
+        raise ValueError(pairwise_energy_error + " (Array has wrong number of dimensions.)")
+    return (np.ascontiguousarray(unary_a, dtype=np.int64),
+            np.ascontiguousarray(pairwise_a, dtype=np.int64))
+
+
 class LatticeFlow(object):
     """An interface to the lattice flow solvers (pyx:82-302)."""
 
@@ -132,65 +197,8 @@ class LatticeFlow(object):
                              np.ascontiguousarray(pairwise_energy, dtype=np.int64))
 
     def _add_int_energy(self, unary_energy, pairwise_energy):
-        unary_energy_error = (
-            "Unary energy must be either a 2d array with first dimension "
-            "matching the number of nodes and second dimension containing E0 and E1, "
-            "or (#dimensions + 1) with the first matching the lattice coordinates and "
-            "the last containing E0 and E1.")
-        if unary_energy.ndim == 2:
-            if unary_energy.shape[0] != self.n_nodes:
-                raise ValueError(unary_energy_error + (" (# nodes = %d; # unary_array_nodes = %d)"
-                                                       % (self.n_nodes, unary_energy.shape[0])))
-            if unary_energy.shape[1] != 2:
-                raise ValueError(unary_energy_error + (" (Second dimension has shape %d; 2 required.)"
-                                                       % unary_energy.shape[1]))
-            unary_a = unary_energy
-        elif unary_energy.ndim == len(self._dimensions) + 1:
-            for i in range(len(self._dimensions)):
-                if unary_energy.shape[i] != self._dimensions[i]:
-                    raise ValueError(unary_energy_error + (" (dimension %d = %d; unary_array shape is %d)"
-                                                           % (i, self._dimensions[i], unary_energy.shape[i])))
-            if unary_energy.shape[-1] != 2:
-                raise ValueError(unary_energy_error + (" (Last dimension has shape %d; 2 required.)"
-                                                       % unary_energy.shape[-1]))
-            unary_a = unary_energy.reshape(-1, 2)
-        else:
-            raise ValueError(unary_energy_error + " (wrong number of dimensions)")
-
-        pairwise_energy_error = (
-            "Pairwise energy must be either: (1) a 3d array with first dimension "
-            "matching the number of nodes, second dimension containing the edge direction output, "
-            "and third dimension containing E00, E01, E10, and E11 (with E01 being energy of "
-            "source node off and dest node on.), or (2) a (#dimensions "
-            "+ 2) array with last two dimensions being the same and first dimensions corresponding "
-            "to the lattice coordinates. ")
-        bad_edge_error = (pairwise_energy_error +
-                          " (wrong number of edges specified in second-to-last dimension; got %d, needed %d).")
-        bad_last_error = pairwise_energy_error + " (wrong number of pairwise energy terms; needed 4, got %d."
-        ne = self.nFillingEdges
-        if pairwise_energy.ndim == 3:
-            if pairwise_energy.shape[0] != self.n_nodes:
-                raise ValueError(pairwise_energy_error + (" (# nodes = %d; # nodes in pairwise_energy = %d)"
-                                                          % (self.n_nodes, pairwise_energy.shape[0])))
-            if pairwise_energy.shape[1] != ne:
-                raise ValueError(bad_edge_error % (pairwise_energy.shape[1], ne))
-            if pairwise_energy.shape[2] != 4:
-                raise ValueError(bad_last_error % pairwise_energy.shape[2])
-            pairwise_a = pairwise_energy
-        elif pairwise_energy.ndim == len(self._dimensions) + 2:
-            for i in range(len(self._dimensions)):
-                if pairwise_energy.shape[i] != self._dimensions[i]:
-                    raise ValueError(pairwise_energy_error + (" (dimension %d shape is %d; pairwise_array shape is %d)"
-                                                              % (i, self._dimensions[i], pairwise_energy.shape[i])))
-            if pairwise_energy.shape[-2] != ne:
-                raise ValueError(bad_edge_error % (pairwise_energy.shape[-2], ne))
-            if pairwise_energy.shape[-1] != 4:
-                raise ValueError(bad_last_error % pairwise_energy.shape[-1])
-            pairwise_a = pairwise_energy.reshape(-1, ne, 4)
-        else:
-            raise ValueError(pairwise_energy_error + " (Array has wrong number of dimensions.)")
-        unary_a = np.ascontiguousarray(unary_a, dtype=np.int64)
-        pairwise_a = np.ascontiguousarray(pairwise_a, dtype=np.int64)
+        unary_a, pairwise_a = _energy_arrays(self._dimensions, self.n_nodes, self.nFillingEdges,
+                                             unary_energy, pairwise_energy)
         check(lib().lf_add_energy_potentials(self._h, unary_a.ctypes.data, pairwise_a.ctypes.data))
 
     def addEnergyDevice(self, unary_ptr, pairwise_ptr, stream=None):
