@@ -7,8 +7,13 @@ one synthetic lattice whose energies were ingested into HBM before timing.
 Default workload: 2048x2048 Full2d_8, int32 capacities, contrast-sensitive
 segmentation energy (64 random rectangles, seed 2).  Prints one JSON line with
 the metric value (Medges/s), the roofline of the dominant kernel (the push /
-relabel round, HBM-bound) measured with hipEvents on the solver stream, and the
+relabel round, HBM-bound) measured with hipEvents on the solver stream, the
+same for the exact global-relabel BFS and for the whole solve, and the
 single-thread CPU baseline (the oracle's PRFlow restatement) timed on this host.
+The line also carries the other two parts of the metric, each measured in the
+same run: "parts" holds the 256^3 26-connected min-cut (Medges/s, parity on a
+sample) and the 1024^2 x 32-lambda TV path, plain and learned (--no-parts
+leaves them out).
 """
 from __future__ import annotations
 
@@ -57,6 +62,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="batch workload: keep the default tuning")
     ap.add_argument("--autotune", action="store_true", help="single-lattice workloads: run lf_autotune first")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parts", action="store_true", help="default line: leave out the 3-D and TV-path parts")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (calibrates the FETCH_SIZE / WRITE_SIZE "
@@ -78,6 +84,7 @@ def main():
     from latticeflow_amd import LatticeFlow, workloads as W
 
     shape = tuple(int(x) for x in a.shape.split("x")) if a.shape else None
+    shape_given = shape is not None
     t0 = time.time()
     w, cap_bytes = make_workload(a.workload, shape)
     if a.verbose:
@@ -112,6 +119,9 @@ def main():
     push_s = 0.0
     tiles = 0
     launches = 0
+    bfs_s = 0.0
+    bfs_visits = 0
+    bfs_launches = 0
     t0 = time.perf_counter()
     for _ in range(a.steps):
         lf.run()
@@ -119,6 +129,9 @@ def main():
         push_s += st["push_seconds"]
         tiles += st["tiles_pushed"]
         launches += st["rounds"]
+        bfs_s += st["bfs_device_seconds"]
+        bfs_visits += st["bfs_visits"]
+        bfs_launches += st["bfs_launches"]
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / a.steps
     flow = lf.flow()
@@ -127,6 +140,10 @@ def main():
 
     bytes_alg = tiles * st["tile_nodes"] * st["bytes_per_node"]
     achieved = bytes_alg / push_s / 1e9 if push_s > 0 else 0.0
+    bfs_bytes = bfs_visits * st["bfs_bytes_per_visit"]
+    bfs_achieved = bfs_bytes / bfs_s / 1e9 if bfs_s > 0 else 0.0
+    traffic, traffic_src = measured_traffic("%s %s %s" % (a.workload, "x".join(map(str, shape)), w["kernel"]),
+                                            tuning, with_source=True)
     roofline = {
         "bound": "hbm",
         "kernel": "k_push (push/relabel round)",
@@ -134,11 +151,22 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic("%s %s %s" % (a.workload, "x".join(map(str, shape)), w["kernel"]), tuning),
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "avg_launch_us": round(push_s / max(launches, 1) * 1e6, 2),
         "bytes_per_node": st["bytes_per_node"],
         "nodes_per_launch": round(tiles * st["tile_nodes"] / max(launches, 1), 1),
         "push_share_of_step": round(push_s / a.steps / dt, 3),
+        # the exact global relabel: nodes its tile BFS visits x (k c + 4 k + 4)
+        # bytes, over the device time of its launches (hipEvents per batch)
+        "bfs": {"kernel": "k_bfs_tile (exact global-relabel BFS)", "achieved": round(bfs_achieved, 1),
+                "frac": round(bfs_achieved / HBM_PEAK_GBS, 4), "bytes_per_visit": st["bfs_bytes_per_visit"],
+                "visits_per_launch": round(bfs_visits / max(bfs_launches, 1), 1),
+                "avg_launch_us": round(bfs_s / max(bfs_launches, 1) * 1e6, 2),
+                "bfs_share_of_step": round(bfs_s / a.steps / dt, 3)},
+        # push + BFS algorithmic bytes over the whole solve's wall time
+        "step": {"achieved": round((bytes_alg + bfs_bytes) / a.steps / dt / 1e9, 1),
+                 "frac": round((bytes_alg + bfs_bytes) / a.steps / dt / 1e9 / HBM_PEAK_GBS, 4)},
     }
 
     cpu = None
@@ -158,10 +186,13 @@ def main():
         cedges = W.n_edges(offs, cshape)
         cpu = {"value": round(cedges / tcpu / 1e6, 3), "unit": "Medges/s", "cores": 1, "kind": "port",
                "sample": "%s %s lattice, one single-thread solve of the PRFlow restatement (oracle/lf_oracle.c),"
-                         " %.2f s" % ("full" if cshape == shape else "first-half", "x".join(map(str, cshape)), tcpu)}
+                         " %.2f s" % ("full" if cshape == shape else "first-half", "x".join(map(str, cshape)), tcpu),
+               **host_cpu()}
         if cshape == shape:
             parity = bool(ref["flow"] == flow and np.array_equal(
                 lf.getCut().reshape(-1), ref["smin"].astype(np.int32)))
+        else:
+            parity = sample_parity(w, cshape, cap_bytes, ref)
 
     out = {
         "metric": METRIC,
@@ -186,21 +217,133 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if a.workload == "seg8" and not shape_given and not a.no_parts:
+        del lf
+        out["parts"] = {"blobs26_256x256x256": part_blobs26(a), "tvpath_1024x1024_32": part_tvpath(a)}
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(workload, tuning):
+def host_cpu():
+    """Host CPU model and the cores this process may use."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"host_cpu": model, "nproc": avail}
+
+
+def sample_parity(w, cshape, cap_bytes, ref):
+    """Device solve of the same leading sample the CPU baseline solved, compared
+    with the oracle's flow and S_min on it."""
+    import numpy as np
+    from latticeflow_amd import LatticeFlow
+    cn = int(np.prod(cshape))
+    lf = LatticeFlow(w["kernel"], "LatticeEnergyMinimizer", cshape, numerical_precision=1.0, cap_bytes=cap_bytes)
+    lf.addEnergyInt(w["unary"][:cn], w["pairwise"][:cn])
+    lf.run()
+    return bool(lf.flow() == ref["flow"] and np.array_equal(lf.getCut().reshape(-1), ref["smin"].astype(np.int32)))
+
+
+def part_blobs26(a):
+    """The 256^3 26-connected min-cut of the metric (blobs volume, int64), timed
+    like the default line, with its CPU baseline and sample parity."""
+    import numpy as np
+    import torch
+    from latticeflow_amd import LatticeFlow, workloads as W
+    w, cap_bytes = make_workload("blobs26", None)
+    shape = w["shape"]
+    lf = LatticeFlow(w["kernel"], "LatticeEnergyMinimizer", shape, numerical_precision=1.0, cap_bytes=cap_bytes)
+    r, it, ts, cap = DEFAULT_TUNING["blobs26"]
+    lf.setTuning(rounds_per_relabel=r, local_iters=it, tile_shape=ts, relabel_launches=cap)
+    du = torch.from_numpy(np.ascontiguousarray(w["unary"])).cuda()
+    dp = torch.from_numpy(np.ascontiguousarray(w["pairwise"])).cuda()
+    torch.cuda.synchronize()
+    lf.addEnergyDevice(du.data_ptr(), dp.data_ptr())
+    del du, dp
+    edges = W.n_edges(lf.directions(), shape)
+    steps = max(3, min(a.steps, 10))
+    lf.run()
+    torch.cuda.synchronize()
+    push_s, tiles, launches = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lf.run()
+        st = lf.stats()
+        push_s += st["push_seconds"]
+        tiles += st["tiles_pushed"]
+        launches += st["rounds"]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    achieved = tiles * st["tile_nodes"] * st["bytes_per_node"] / push_s / 1e9 if push_s > 0 else 0.0
+    res = {"workload": "blobs26 256x256x256 Star3d_26 int64", "value": round(edges / dt / 1e6, 3),
+           "unit": "Medges/s", "ms_per_step": round(dt * 1e3, 3), "steps": steps, "edges": edges,
+           "flow": lf.flow(), "k_push_frac": round(achieved / HBM_PEAK_GBS, 4)}
+    del lf
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        cshape = (shape[0] // 2,) + tuple(shape[1:])
+        cn = int(np.prod(cshape))
+        offs = O.edge_order(w["kernel"], cshape)[0]
+        red, alpha = O.reparam(cshape, offs, w["unary"][:cn], w["pairwise"][:cn])
+        tc = time.perf_counter()
+        ref = O.prflow(cshape, offs, red, alpha)
+        tcpu = time.perf_counter() - tc
+        res["cpu_baseline"] = {"value": round(W.n_edges(offs, cshape) / tcpu / 1e6, 3), "unit": "Medges/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "first-half %s, one single-thread PRFlow-restatement solve, %.2f s"
+                                         % ("x".join(map(str, cshape)), tcpu), **host_cpu()}
+        res["parity_vs_cpu_sample"] = sample_parity(w, cshape, cap_bytes, ref)
+    return res
+
+
+def part_tvpath(a):
+    """The TV-path part of the metric: 1024^2 Voronoi image, Full2d_8, 32
+    lambdas log-spaced in [0.01, 2]; plain and learned wall time (one timed
+    path each after a warm-up), their output difference, and the oracle's
+    parity on two of the lambdas."""
+    import argparse
+    ns = argparse.Namespace(**vars(a))
+    ns.steps, ns.warmup, ns.shape = 1, 1, ""
+    return tvpath_record(ns)
+
+
+def measured_traffic(workload, tuning, with_source=False):
     """HBM bytes per k_push launch from the calibrated FETCH_SIZE / WRITE_SIZE
     counter runs recorded in profiles/traffic.json, when they were taken on
     this workload at this tuning; else None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
+    none = (None, None) if with_source else None
     try:
         rec = json.load(open(path)).get(workload)
     except (OSError, ValueError):
-        return None
+        return none
     if not rec or any(tuning.get(k) != v for k, v in rec.get("tuning", {}).items()):
-        return None
+        return none
+    # the counters were taken on this kernel source only
+    src = os.path.join(ROOT, "latticeflow_amd", "csrc", "lf_mincut.hip")
+    if rec.get("kernel_source_sha256") != _sha256(src):
+        return none
+    if with_source:
+        return rec["bytes_per_launch"], ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this command on this "
+                                         "kernel source, calibrated by a copy kernel (%s)" % rec.get("source", ""))
     return rec["bytes_per_launch"]
+
+
+def _sha256(path):
+    import hashlib
+    try:
+        return hashlib.sha256(open(path, "rb").read()).hexdigest()
+    except OSError:
+        return None
 
 
 def copy_probe():
@@ -293,7 +436,8 @@ def bench_batch(a):
         cpu = {"value": round(sedges / tcpu / 1e6, 3), "unit": "Medges/s", "cores": threads, "kind": "port",
                "sample": "%d of the %d lattices (every %dth, %d edges), one PRFlow-restatement solve each "
                          "(oracle/lf_oracle.c) on a pool of %d threads, %.2f s"
-                         % (len(prep), count, max(1, count // 128), sedges, threads, tcpu)}
+                         % (len(prep), count, max(1, count // 128), sedges, threads, tcpu),
+               **host_cpu()}
     hc, wc = bt.canvas()
     out = {
         "metric": "Batched min-cut Medges/s (%d Full2d_8 lattices, sides 64..1024)" % count,
@@ -316,6 +460,10 @@ def bench_tvpath(a):
     log-spaced in [0.01, 2], all lambdas solved together on the device.  Plain:
     divide and conquer with the mean threshold; learned: thresholds from the
     shipped net's predictions (same output).  value = plain path time."""
+    print(json.dumps(tvpath_record(a, full_line=True)), flush=True)
+
+
+def tvpath_record(a, full_line=False):
     import numpy as np
     import torch
     torch.zeros(1, device="cuda")
@@ -354,9 +502,15 @@ def bench_tvpath(a):
                "cores": 1, "kind": "port",
                "sample": "lambdas %s on the full %s image with the oracle's divide and conquer, %.2f s for %d "
                          "lambdas, scaled to 32" % ([round(float(lams[i]), 4) for i in pick],
-                                                     "x".join(map(str, shape)), tcpu, len(pick))}
+                                                     "x".join(map(str, shape)), tcpu, len(pick)),
+               **host_cpu()}
         rel = float(np.abs(R[pick] - Rr).max() / np.abs(Rr).max())
         parity = {"max_rel_err": rel, "within_1e-9": rel <= 1e-9}
+    if not full_line:
+        return {"workload": "tvpath %s Full2d_8 Voronoi(256 cells)+N(0,0.1), 32 lambdas log[0.01,2]"
+                % "x".join(map(str, shape)), "plain_seconds": round(dt, 3), "learned_seconds": round(dl, 3),
+                "speedup_learned_vs_plain": round(dt / dl, 3), "max_rel_diff_learned_vs_plain": same,
+                "parity_vs_cpu": parity, "guided": gs, "cpu_baseline": cpu}
     out = {
         "metric": "TV-path wall time (exact, 32 lambdas), plain divide and conquer",
         "value": round(dt, 3), "unit": "s", "n_gpus": 1, "steps": len(times), "warmup": a.warmup,
@@ -370,7 +524,7 @@ def bench_tvpath(a):
                                "net": "3-layer 3x3 conv, 16 channels, bf16 MFMA (latticeflow_amd/data/tvnet_params.npy)"}},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -64,6 +64,11 @@ typedef struct {
   double push_seconds;   /* device time of all push launches (hipEvents on the solver stream) */
   double bfs_seconds;    /* wall time in global relabels (BFS + delivery + counting) */
   double flood_seconds;  /* wall time of the S_min flood */
+  int64_t bfs_visits;          /* node visits of the exact global-relabel BFS (a tile's pull pass over
+                                  its nodes plus every node its relaxation steps list) */
+  int64_t bfs_bytes_per_visit; /* algorithmic HBM bytes per BFS visit: k residuals, k neighbour
+                                  labels and the node's label (k c + 4 k + 4) */
+  double bfs_device_seconds;   /* device time of the BFS launches (hipEvents around each batch) */
 } lf_stats;
 
 /* Statistics of a guided divide and conquer (lf_tv_path_guided). */
@@ -214,6 +219,39 @@ int lf_tv_path(const char *kernel, int64_t nx, int64_t ny, const double *f, int6
 int lf_tv_path_guided(const char *kernel, int64_t nx, int64_t ny, const double *f, int64_t n_lambda,
                       const double *lambdas, const float *d_pred, double delta, int max_repairs, double *R,
                       lf_stats *stats, lf_guided_stats *gstats, void *stream);
+
+/* ---- event-driven regularization path ------------------------------------
+ * The whole path of one image over lambda in [0, lambda_max], as the
+ * reference's TVSolver builds it (TVSolver::run, tv_solver.hpp:114-149;
+ * _constructInitialRegPathsFromSolvedLattice :300-564; TVRegPathSegment,
+ * tv_regpath_node.hpp:67-1028): the exact solution at lambda_max, then level
+ * sets that join in closed form and split where a minimum cut says so, down
+ * to lambda = 0.  Every minimum cut runs on the device; the checks of
+ * overlapping level sets are batched as `layers` stacked copies of the
+ * lattice (0 = 4; else 2..64).  lf_tvpath_eval then gives the path at any
+ * lambdas in [0, lambda_max] (getRegularizationPath, tv_solver.hpp:181-244:
+ * slice l for lambdas[l], negative lambdas count as 0, a lambda above
+ * lambda_max is LF_ERR_OUT_OF_RANGE), the same output as lf_tv_path's with
+ * lambda_max the largest lambda of the list. */
+typedef struct lf_tvpath lf_tvpath;
+typedef struct lf_tvpath_stats {
+  int64_t initial_regions;   /* level sets at lambda_max */
+  int64_t segments;          /* level sets of the path, each alive on an interval */
+  int64_t joins;             /* join events */
+  int64_t splits;            /* split events */
+  int64_t checks;            /* level-set checks (one minimum cut each) */
+  int64_t device_calls;      /* batched check launches */
+  int64_t dinkelbach_steps;  /* further checks that located split points */
+  int64_t passes;            /* event passes (one more per split found) */
+  double initial_seconds;    /* the exact solution at lambda_max */
+  double check_seconds;      /* batched checks */
+  double seconds;            /* the whole construction */
+} lf_tvpath_stats;
+int lf_tvpath_create(lf_tvpath **out, const char *kernel, int64_t nx, int64_t ny, const double *f,
+                     double lambda_max, int layers);
+int lf_tvpath_eval(const lf_tvpath *p, int64_t n_lambda, const double *lambdas, double *R);
+int lf_tvpath_get_stats(const lf_tvpath *p, lf_tvpath_stats *stats);
+int lf_tvpath_destroy(lf_tvpath *p);
 
 /* ---- learned TV predictor ---------------------------------------------------
  * No reference counterpart: a 3-layer 3x3 convolutional net (16 channels, BF16

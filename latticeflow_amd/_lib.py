@@ -10,7 +10,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblatticeflow_amd.so")
+# LATTICEFLOW_AMD_LIBRARY selects another build of the library (same ABI)
+LIB_PATH = os.environ.get("LATTICEFLOW_AMD_LIBRARY") or os.path.join(_HERE, "liblatticeflow_amd.so")
 
 LF_OK = 0
 LF_ERR_BAD_NAME = -1
@@ -51,7 +52,9 @@ class lf_stats(ctypes.Structure):
                 ("tiles_pushed", ctypes.c_int64), ("tile_nodes", ctypes.c_int64),
                 ("bytes_per_node", ctypes.c_int64), ("reserved", ctypes.c_int64),
                 ("seconds", ctypes.c_double), ("push_seconds", ctypes.c_double),
-                ("bfs_seconds", ctypes.c_double), ("flood_seconds", ctypes.c_double)]
+                ("bfs_seconds", ctypes.c_double), ("flood_seconds", ctypes.c_double),
+                ("bfs_visits", ctypes.c_int64), ("bfs_bytes_per_visit", ctypes.c_int64),
+                ("bfs_device_seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -61,6 +64,17 @@ class lf_guided_stats(ctypes.Structure):
     _fields_ = [("regions", ctypes.c_int64), ("repairs", ctypes.c_int64), ("failing_arcs", ctypes.c_int64),
                 ("layers_merged", ctypes.c_int64), ("fallback", ctypes.c_int64),
                 ("partition_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class lf_tvpath_stats(ctypes.Structure):
+    _fields_ = [("initial_regions", ctypes.c_int64), ("segments", ctypes.c_int64), ("joins", ctypes.c_int64),
+                ("splits", ctypes.c_int64), ("checks", ctypes.c_int64), ("device_calls", ctypes.c_int64),
+                ("dinkelbach_steps", ctypes.c_int64), ("passes", ctypes.c_int64),
+                ("initial_seconds", ctypes.c_double), ("check_seconds", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -117,6 +131,10 @@ _SIG = {
     "lf_tv_path": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.POINTER(lf_stats)]),
     "lf_tv_path_guided": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.c_double, _I, _P,
                                ctypes.POINTER(lf_stats), ctypes.POINTER(lf_guided_stats), _P]),
+    "lf_tvpath_create": (_I, [ctypes.POINTER(_P), _CP, _I64, _I64, _P, ctypes.c_double, _I]),
+    "lf_tvpath_eval": (_I, [_P, _I64, _P, _P]),
+    "lf_tvpath_get_stats": (_I, [_P, ctypes.POINTER(lf_tvpath_stats)]),
+    "lf_tvpath_destroy": (_I, [_P]),
 }
 
 _lib = None

@@ -22,6 +22,7 @@
 #include "lf_engine.hpp"
 #include "lf_kernels.hpp"
 #include "lf_tv.hpp"
+#include "lf_tvpath.hpp"
 
 using namespace lfamd;
 
@@ -820,6 +821,52 @@ int lf_tv_path_guided(const char* kernel, int64_t nx, int64_t ny, const double* 
     if (max_repairs < 0) fail(LF_ERR_BAD_ARG, "max_repairs must be >= 0");
     tv_path_impl(kernel, nx, ny, f, n_lambda, lambdas, R, stats, d_pred, delta, max_repairs, gstats);
   });
+}
+
+// ---- event-driven regularization path --------------------------------------------
+
+struct lf_tvpath {
+  std::unique_ptr<lfamd::TvPath> path;
+};
+
+int lf_tvpath_create(lf_tvpath** out, const char* kernel, int64_t nx, int64_t ny, const double* f,
+                     double lambda_max, int layers) {
+  return guarded([&] {
+    if (!out) fail(LF_ERR_BAD_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<lf_tvpath> p(new lf_tvpath);
+    p->path.reset(new TvPath(kernel, nx, ny, f, lambda_max, layers));
+    *out = p.release();
+  });
+}
+
+int lf_tvpath_eval(const lf_tvpath* p, int64_t n_lambda, const double* lambdas, double* R) {
+  return guarded([&] {
+    if (!p) fail(LF_ERR_BAD_ARG, "null handle");
+    p->path->eval(n_lambda, lambdas, R);
+  });
+}
+
+int lf_tvpath_get_stats(const lf_tvpath* p, lf_tvpath_stats* out) {
+  return guarded([&] {
+    if (!p || !out) fail(LF_ERR_BAD_ARG, "null argument");
+    const TvPathStats& s = p->path->stats();
+    out->initial_regions = s.initial_regions;
+    out->segments = s.segments;
+    out->joins = s.joins;
+    out->splits = s.splits;
+    out->checks = s.checks;
+    out->device_calls = s.device_calls;
+    out->dinkelbach_steps = s.dinkelbach_steps;
+    out->passes = s.passes;
+    out->initial_seconds = s.initial_seconds;
+    out->check_seconds = s.check_seconds;
+    out->seconds = s.seconds;
+  });
+}
+
+int lf_tvpath_destroy(lf_tvpath* p) {
+  return guarded([&] { delete p; });
 }
 
 // ---- learned TV predictor ------------------------------------------------------

@@ -82,6 +82,16 @@ class MinCutEngine {
   virtual void set_tv_network(const int64_t* uf, const int64_t* pwf, int layers) = 0;
   virtual void pf_values_device(int64_t* d_out, hipStream_t s) = 0;
   virtual void pf_cut_device(int32_t* d_out, hipStream_t s) = 0;  // r < 0
+  // Region id and flow-consistent value r of every node after a divide and
+  // conquer (device pointers).
+  virtual void pf_regions_device(int32_t* d_reg, int64_t* d_r, hipStream_t s) = 0;
+  // Level-set check of many regions at once (host arrays of N entries): reg[v]
+  // the region of node v (0..nreg-1, -1 = not checked), r[v] its value; each
+  // region is cut at ceil(mean r) with the arcs to other regions cut, and
+  // cnt_low[k] receives the number of region k's nodes on the low side of the
+  // minimal cut (low[v] = 1 for them).  Region k is a level set of its own
+  // problem when cnt_low[k] is 0 or its size.
+  virtual void pf_check_regions(const int64_t* r, const int32_t* reg, int nreg, int64_t* cnt_low, uint8_t* low) = 0;
   // Replace the base network (host arrays: red[N], alpha[N*k] node-major, lattice order).
   virtual void set_base_network(const int64_t* red, const int64_t* alpha) = 0;
   // Batched lattices on a 2-D canvas (arcs between lattices stay zero).

@@ -190,11 +190,6 @@ __device__ inline void tile_coords(int t, int& z, int& y, int& x) {
   x = t % G::TW;
 }
 
-// Append tile t to a push work list once per round (its flag dedups).
-__device__ inline void wl_append(int* flags, int* list, unsigned* cnt, int t) {
-  if (atomicExch(&flags[t], 1) == 0) list[atomicAdd(cnt, 1u)] = t;
-}
-
 // One push / relabel round over the tiles of the work list; a fixed grid of
 // resident blocks walks the list, so a round over few active tiles costs
 // little more than those tiles.
@@ -205,28 +200,28 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   __shared__ T s_ex[NT];
   __shared__ int s_h[G::NH];
   __shared__ unsigned short s_list[NT];
-  // bit i: residual plane i changed; top bit: excess or label changed
-  using DirtyT = typename std::conditional<(K <= 31), unsigned int, unsigned long long>::type;
-  constexpr DirtyT kTouched = (DirtyT)1 << (8 * sizeof(DirtyT) - 1);
-  __shared__ DirtyT s_dirty[NT];
   __shared__ int s_cnt[2];
   __shared__ int s_tile;
   __shared__ unsigned int s_nbr;  // bit 9 (sz+1) + 3 (sy+1) + (sx+1): neighbouring tile received flow
   __shared__ unsigned int s_mail;  // sides of this tile whose neighbours sent flow (p.mail)
-  static_assert(K <= 63, "dirty words hold 63 planes");
 
   if (blockIdx.x == 0 && threadIdx.x == 0) *p.wl_cnt_zero = 0;
   const int ntodo = (int)*p.wl_cnt_in;
+  // thread 0 loads the id of its block's next tile while the current one is
+  // processed
+  int next_tile = (threadIdx.x == 0 && (int)blockIdx.x < ntodo) ? p.wl_in[blockIdx.x] : -1;
+  unsigned long long ntiles_done = 0;
   for (int li = blockIdx.x; li < ntodo; li += gridDim.x) {
     if (threadIdx.x == 0) {
-      const int t = p.wl_in[li];
+      const int t = next_tile;
+      next_tile = li + (int)gridDim.x < ntodo ? p.wl_in[li + gridDim.x] : -1;
       s_tile = t;
       p.flag_in[t] = 0;
       s_cnt[0] = 0;
       s_cnt[1] = 0;
       s_nbr = 0;
       s_mail = atomicExch(&p.mail[t], 0u);
-      atomicAdd(p.tiles, 1ull);
+      ++ntiles_done;
     }
     __syncthreads();
     const int tile = s_tile;
@@ -243,6 +238,18 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
     {
       T r[G::NPT][K + 1];
       bool in[G::NPT];
+      // labels of the tile and its halo, loaded in the same batch
+      constexpr int NHI = (G::NH + NTHREADS - 1) / NTHREADS;
+      int hv[NHI];
+      bool hin[NHI];
+#pragma unroll
+      for (int k = 0; k < NHI; ++k) {
+        const int q = threadIdx.x + k * NTHREADS;
+        const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
+        const int gz = z0 + hz - G::RZ, gy = y0 + hy - G::RY, gx = x0 + hx - G::RX;
+        hin[k] = q < G::NH && gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W;
+        hv[k] = p.h[hin[k] ? ((long long)gz * p.H + gy) * p.W + gx : 0];
+      }
 #pragma unroll
       for (int j = 0; j < G::NPT; ++j) {
         int z, y, x;
@@ -253,28 +260,62 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         for (int i = 0; i < K; ++i) r[j][i] = p.res[i * N + g];
         r[j][K] = p.ex[g];
       }
+      // kernels of at most HOIST directions take their mailbox slots (the
+      // sides the mail bits name, with atomic exchanges: a sender of this
+      // round may still add to a slot; what it adds later waits for the next
+      // round) in the same batch as the staging loads
+      T a[G::NPT][K <= HOIST ? K : 1];
+      if constexpr (K <= HOIST) {
+        const unsigned mail = s_mail;
+#pragma unroll
+        for (int j = 0; j < G::NPT; ++j) {
+          int z, y, x;
+          tile_coords<G>(threadIdx.x + j * NTHREADS, z, y, x);
+          const int gz = z0 + z, gy = y0 + y, gx = x0 + x;
+#pragma unroll
+          for (int i = 0; i < K; ++i) {
+            const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+            const int lz = z + dz, ly = y + dy, lx = x + dx;
+            const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
+            const int side = (lz < 0 ? 0 : (lz >= G::TD ? 2 : 1)) * 9 + (ly < 0 ? 0 : (ly >= G::TH ? 2 : 1)) * 3 +
+                             (lx < 0 ? 0 : (lx >= G::TW ? 2 : 1));
+            a[j][i] = 0;
+            if (side != 13 && ((mail >> side) & 1u) && in[j] && nz >= 0 && nz < p.D && ny >= 0 && ny < p.H &&
+                nx >= 0 && nx < p.W)
+              a[j][i] = gtake(&p.mbox[(long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx]);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NHI; ++k) {
+        const int q = threadIdx.x + k * NTHREADS;
+        if (q < G::NH) s_h[q] = hin[k] ? hv[k] : INF_H;
+      }
 #pragma unroll
       for (int j = 0; j < G::NPT; ++j) {
         const int t = threadIdx.x + j * NTHREADS;
+        T add = 0;
 #pragma unroll
-        for (int i = 0; i < K; ++i) s_res[i * NT + t] = in[j] ? r[j][i] : (T)0;
-        s_ex[t] = in[j] ? r[j][K] : (T)0;
-        s_dirty[t] = 0;
+        for (int i = 0; i < K; ++i) {
+          T v = in[j] ? r[j][i] : (T)0;
+          if constexpr (K <= HOIST) {
+            v += a[j][i];
+            add += a[j][i];
+          }
+          s_res[i * NT + t] = v;
+        }
+        s_ex[t] = (in[j] ? r[j][K] : (T)0) + add;
       }
     }
     // collect the flow that neighbouring tiles pushed into this tile, from the
     // sides the mail bits name: every slot is taken with an atomic exchange
     // (a sender of this round may still add to it; what it adds later waits
     // for the next round), all exchanges of a thread in flight at once
-    if (s_mail) {
+    if (K > HOIST && s_mail) {
       const unsigned mail = s_mail;
       T add_e[G::NPT];
-      DirtyT dirty[G::NPT];
 #pragma unroll
-      for (int j = 0; j < G::NPT; ++j) {
-        add_e[j] = 0;
-        dirty[j] = 0;
-      }
+      for (int j = 0; j < G::NPT; ++j) add_e[j] = 0;
 #pragma unroll
       for (int c = 0; c < K; c += HOIST) {
         T a[G::NPT][HOIST];
@@ -306,26 +347,11 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
             const int i = c + u;
             add_e[j] += a[j][u];
             s_res[i * NT + t] += a[j][u];  // only this thread touches node t's planes while staging
-            dirty[j] |= ((DirtyT)1 << i) | kTouched;
           }
         }
       }
 #pragma unroll
-      for (int j = 0; j < G::NPT; ++j) {
-        if (!dirty[j]) continue;
-        const int t = threadIdx.x + j * NTHREADS;
-        s_ex[t] += add_e[j];
-        s_dirty[t] = dirty[j];
-      }
-    }
-#pragma unroll
-    for (int q0 = 0; q0 < G::NH; q0 += NTHREADS) {
-      const int q = q0 + threadIdx.x;
-      const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
-      const int gz = z0 + hz - G::RZ, gy = y0 + hy - G::RY, gx = x0 + hx - G::RX;
-      const bool in = q < G::NH && gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W;
-      const int v = p.h[in ? ((long long)gz * p.H + gy) * p.W + gx : 0];
-      if (q < G::NH) s_h[q] = in ? v : INF_H;
+      for (int j = 0; j < G::NPT; ++j) s_ex[threadIdx.x + j * NTHREADS] += add_e[j];
     }
     __syncthreads();
 
@@ -367,7 +393,6 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         // node t's residuals, and a receiver's increase lands on an arc that t
         // does not push along, since both ends cannot be lower than the other)
         T e = e0;
-        DirtyT dirty = 0;
 #pragma unroll
         for (int c = 0; c < K; c += HOIST) {
           int hn[HOIST];
@@ -388,13 +413,11 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
                 const T d = r < e ? r : e;
                 s_res[i * NT + t] = r - d;
                 e -= d;
-                dirty |= (DirtyT)1 << i;
                 const int lz = z + dz, ly = y + dy, lx = x + dx;
                 if (lz >= 0 && lz < G::TD && ly >= 0 && ly < G::TH && lx >= 0 && lx < G::TW) {
                   const int tn = t + (dz * G::TH + dy) * G::TW + dx;
                   lds_add(&s_res[((i + K / 2) % K) * NT + tn], d);
                   lds_add(&s_ex[tn], d);
-                  atomicOr(&s_dirty[tn], ((DirtyT)1 << ((i + K / 2) % K)) | kTouched);
                 } else {
                   const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
                   gadd(&p.mbox[(long long)i * N + g], d);
@@ -406,10 +429,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
             }
           }
         }
-        if (e != e0) {
-          lds_add(&s_ex[t], e - e0);
-          atomicOr(&s_dirty[t], dirty | kTouched);
-        }
+        if (e != e0) lds_add(&s_ex[t], e - e0);
       }
       __syncthreads();
 
@@ -436,15 +456,14 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
 #pragma unroll
           for (int u = 0; u < HOIST && c + u < K; ++u) m = (rr[u] > 0 && hn[u] < m) ? hn[u] : m;
         }
-        if (m >= hv) {
-          s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
-          atomicOr(&s_dirty[t], kTouched);
-        }
+        if (m >= hv) s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
       }
       __syncthreads();
     }
 
-    // write back what changed (residual planes, excess and label of touched nodes)
+    // write back the tile's residual planes, excess, labels and arc masks
+    // (whole: tracking which words changed measured slower, nearly every node
+    // of a processed tile changes in a round)
     bool act = false;
 #pragma unroll
     for (int j = 0; j < G::NPT; ++j) {
@@ -452,46 +471,63 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
       int z, y, x;
       tile_coords<G>(t, z, y, x);
       if (z >= ld || y >= lh || x >= lw) continue;
-      const DirtyT dm = s_dirty[t];
       const T e = s_ex[t];
       const int hv = s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)];
-      if (dm) {
-        const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+      const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
+      MaskT<K> m = 0;
 #pragma unroll
-        for (int i = 0; i < K; ++i)
-          if (dm & ((DirtyT)1 << i)) p.res[i * N + g] = s_res[i * NT + t];
-        p.ex[g] = e;
-        p.h[g] = hv;
-        if (dm & ~kTouched) {
-          MaskT<K> m = 0;
-#pragma unroll
-          for (int i = 0; i < K; ++i)
-            if (s_res[i * NT + t] > 0) m |= (MaskT<K>)1 << i;
-          reinterpret_cast<MaskT<K>*>(p.mask)[g] = m;
-        }
+      for (int i = 0; i < K; ++i) {
+        const T r = s_res[i * NT + t];
+        p.res[i * N + g] = r;
+        if (r > 0) m |= (MaskT<K>)1 << i;
       }
+      p.ex[g] = e;
+      p.h[g] = hv;
+      reinterpret_cast<MaskT<K>*>(p.mask)[g] = m;
       if (e > 0 && hv < INF_H) act = true;
     }
+    const unsigned nbr_sides = s_nbr;  // final: the last push phase ended at a barrier
     const bool self_active = __syncthreads_or(act);
-    if (threadIdx.x == 0) {
-      // next round: this tile while it holds active nodes, and every tile
-      // whose mailbox it filled
-      unsigned m = s_nbr | (self_active ? 1u << 13 : 0u);
-      while (m) {
-        const int b = __builtin_ctz(m);
-        m &= m - 1;
-        const int nz = tz + b / 9 - 1, ny = ty + (b / 3) % 3 - 1, nx = tx + b % 3 - 1;
-        if (nz < 0 || nz >= p.ntz || ny < 0 || ny >= p.nty || nx < 0 || nx >= p.ntx) continue;
+    // next round: this tile while it holds active nodes, and every tile whose
+    // mailbox it filled.  Lane b of the last wave handles side b, so the
+    // appends of all sides are in flight together (one after the other they
+    // were a chain of dependent atomics at the end of every tile), and the
+    // list counter takes one atomic per tile (a single address every tile of
+    // the round adds to)
+    bool claim = false;
+    int claimed_tile = 0;
+    if (threadIdx.x >= NTHREADS - 64 && threadIdx.x < NTHREADS - 64 + 27) {
+      const int b = threadIdx.x - (NTHREADS - 64);
+      const unsigned m = nbr_sides | (self_active ? 1u << 13 : 0u);
+      const int nz = tz + b / 9 - 1, ny = ty + (b / 3) % 3 - 1, nx = tx + b % 3 - 1;
+      if (((m >> b) & 1u) && nz >= 0 && nz < p.ntz && ny >= 0 && ny < p.nty && nx >= 0 && nx < p.ntx) {
         const int nt = (nz * p.nty + ny) * p.ntx + nx;
         if (b != 13) {  // seen from nt, this tile is at the mirrored offset
           atomicOr(&p.mail[nt], 1u << (26 - b));
           atomicOr(&p.dmail[nt], 1u << (26 - b));
         }
-        wl_append(p.flag_out, p.wl_out, p.wl_cnt_out, nt);
+        claim = atomicExch(&p.flag_out[nt], 1) == 0;
+        claimed_tile = nt;
       }
     }
-    __syncthreads();
+    if (threadIdx.x >= NTHREADS - 64) {
+      // one counter atomic for all the tiles this one appends
+      const unsigned long long won = __ballot(claim);
+      const int lane = threadIdx.x & 63;
+      unsigned base = 0;
+      if (won) {
+        const int first = __builtin_ctzll(won);
+        if (lane == first) base = atomicAdd(p.wl_cnt_out, (unsigned)__popcll(won));
+        base = __shfl(base, first, 64);
+        if (claim) p.wl_out[base + __popcll(won & ((1ull << lane) - 1))] = claimed_tile;
+      }
+    }
+    // no barrier here: the next tile's first barrier follows thread 0's
+    // prologue; the appends above use only registers (nbr_sides was read
+    // before the last barrier), while thread 0 may already reset s_nbr
   }
+  // processed tiles: one atomic per block, not per tile
+  if (threadIdx.x == 0 && ntiles_done) atomicAdd(p.tiles, ntiles_done);
 }
 
 // Work list of the tiles whose flag is set (the first round after a relabel).
@@ -602,6 +638,9 @@ struct BP {
   int* flag_in;
   int* flag_out;
   unsigned long long* counter;
+  // node visits of the exact BFS (the pull pass over a tile's nodes plus every
+  // listed node of its relaxation steps), one slot per XCD; null: not counted
+  unsigned long long* visits;
   DirTab dirs;
 };
 
@@ -617,9 +656,11 @@ __device__ inline bool bborder(int t) {
   return z < B::RZ || z >= B::TD - B::RZ || y < B::RY || y >= B::TH - B::RY || x < B::RX || x >= B::TW - B::RX;
 }
 
-// Flag the 3^d - 1 neighbouring tiles and count a border change.
+// Flag the 3^d - 1 neighbouring tiles and record a border change (the host
+// only tests the counter for zero, so a plain store serves every tile of the
+// launch without an atomic on one address).
 __device__ inline void flag_around(const BP& p, int tile) {
-  atomicAdd(p.counter, 1ull);
+  *p.counter = 1ull;
   const int tx = tile % p.ntx, ty = (tile / p.ntx) % p.nty, tz = tile / (p.ntx * p.nty);
   for (int a = -1; a <= 1; ++a)
     for (int b = -1; b <= 1; ++b)
@@ -727,10 +768,12 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
   // one barrier per step: step k reads list k&1 with count s_n[k%3], appends
   // to list (k+1)&1 with count s_n[(k+1)%3], and clears s_n[(k+2)%3], which
   // every thread last read before the previous step's barrier
+  unsigned long long nvis = (unsigned long long)ld * lh * lw;  // the pull pass
   for (int k = 0;; ++k) {
     const int cur = k & 1, c_in = k % 3, c_out = (k + 1) % 3;
     const int n = s_n[c_in];
     if (n == 0) break;
+    nvis += (unsigned long long)n;
     if (threadIdx.x == 0) s_n[(k + 2) % 3] = 0;
     // n counts every claim of the previous step, also those past LCAP
     const bool scan = n > LCAP;
@@ -785,6 +828,7 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
       if (bborder<B>(t)) border = true;
     }
   }
+  if (threadIdx.x == 0 && p.visits) atomicAdd(&p.visits[blockIdx.x & 7], nvis);
   if (__syncthreads_or(border) && threadIdx.x == 0) flag_around(p, tile);
 }
 
@@ -1953,6 +1997,13 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_cutarcs(PFP f, KP<T> p) {
   if (cut) reinterpret_cast<MaskT<K>*>(p.mask)[v] &= (MaskT<K>)~cut;
 }
 
+// a region check: regions [0, nchecked) take part (state 1), the rest is final
+__global__ __launch_bounds__(NTHREADS) void k_pf_init_check(PFP f, int nreg, int nchecked) {
+  const int k = blockIdx.x * NTHREADS + threadIdx.x;
+  if (k < nreg) f.tab[k].state = k < nchecked ? 1 : 2;
+  if (k == 0) *f.nreg = nreg;
+}
+
 __global__ __launch_bounds__(NTHREADS) void k_pf_cut(PFP f, int* __restrict__ out) {
   const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
   if (v >= f.N) return;
@@ -2321,6 +2372,9 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipHostMalloc(&host_small_, 64, hipHostMallocDefault));
     LF_HIP(hipEventCreate(&ev_[0]));
     LF_HIP(hipEventCreate(&ev_[1]));
+    LF_HIP(hipEventCreate(&evb_[0]));
+    LF_HIP(hipEventCreate(&evb_[1]));
+    LF_HIP(hipMalloc(&bvisits_, 8 * sizeof(unsigned long long)));
     reset_energies();
   }
   ~Engine() override {
@@ -2334,6 +2388,9 @@ class Engine final : public MinCutEngine {
     hipFree(g_preg_); hipFree(g_parent_); hipFree(g_idmap_); hipFree(g_merged_); hipFree(g_layer_bad_); hipFree(g_val_);
     hipEventDestroy(ev_[0]);
     hipEventDestroy(ev_[1]);
+    hipEventDestroy(evb_[0]);
+    hipEventDestroy(evb_[1]);
+    hipFree(bvisits_);
     for (auto& kv : graphs_) hipGraphExecDestroy(kv.second);
     hipStreamDestroy(s_);
   }
@@ -2543,13 +2600,19 @@ class Engine final : public MinCutEngine {
 
   void begin_stats() {
     stats = lf_stats{};
-    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(1), dim3(NTHREADS), 0, s_, tiles_ptr(), 0ull, 1LL);
+    bfs_time_pending_ = false;
+    fill_many({{u32(tiles_ptr()), 0u, 2}, {u32(bvisits_), 0u, 16}});
   }
   void end_stats() {
     stats.tile_nodes = grid_[variant_].nt;
     stats.bytes_per_node = 2 * ((long long)K * sizeof(T) + sizeof(T) + 4);
+    stats.bfs_bytes_per_visit = (long long)K * sizeof(T) + 4LL * K + 4;
     LF_HIP(hipMemcpy(host_small_, tiles_ptr(), sizeof(unsigned long long), hipMemcpyDeviceToHost));
     stats.tiles_pushed = (long long)*reinterpret_cast<unsigned long long*>(host_small_);
+    unsigned long long v[8];
+    LF_HIP(hipMemcpy(v, bvisits_, sizeof(v), hipMemcpyDeviceToHost));
+    stats.bfs_visits = 0;
+    for (unsigned long long x : v) stats.bfs_visits += (long long)x;
   }
 
   void run(const lf_tuning& tun) override {
@@ -2915,6 +2978,54 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipGetLastError());
     sync_to(s);
   }
+  void pf_regions_device(int32_t* d_reg, int64_t* d_r, hipStream_t s) override {
+    if (!pf_solved) fail(LF_ERR_STATE, "no divide and conquer has run");
+    LF_HIP(hipMemcpyAsync(d_reg, pf_reg_, sizeof(int) * N_, hipMemcpyDeviceToDevice, s_));
+    LF_HIP(hipMemcpyAsync(d_r, pf_r_, sizeof(long long) * N_, hipMemcpyDeviceToDevice, s_));
+    sync_to(s);
+  }
+
+  // Every region of reg (ids 0..nreg-1, -1: not checked) is cut at ceil of the
+  // mean of its values r, with the arcs to other regions cut: the number of
+  // its nodes on the low side of the minimal cut goes to cnt_low (0 when the
+  // values span at most one unit), each node's side to low.  A region is a
+  // level set of its own problem exactly when that side is empty or whole.
+  void pf_check_regions(const int64_t* r, const int32_t* reg, int nreg, int64_t* cnt_low, uint8_t* low) override {
+    if (nreg < 1) return;
+    PfMode pm{pf_mode_};
+    const auto t0 = std::chrono::steady_clock::now();
+    load_working();
+    ensure_pf_buffers(std::max<long long>(nreg + 1, 1024));
+    std::vector<int> regd((size_t)N_);
+    for (long long v = 0; v < N_; ++v) regd[v] = reg[v] < 0 ? nreg : reg[v];
+    LF_HIP(hipMemcpyAsync(pf_r_, r, sizeof(long long) * N_, hipMemcpyHostToDevice, s_));
+    LF_HIP(hipMemcpyAsync(pf_reg_, regd.data(), sizeof(int) * N_, hipMemcpyHostToDevice, s_));
+    PFP pf = pf_params();
+    KP<T> p = params();
+    clear_table();
+    zero_int(pf.err, 1);
+    hipLaunchKernelGGL((k_pf_init_check), dim3(blocks_for(nreg + 1)), dim3(NTHREADS), 0, s_, pf, nreg + 1, nreg);
+    hipLaunchKernelGGL((k_pf_cutarcs<K, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, p);
+    LF_HIP(hipGetLastError());
+    long long predicted = 0;
+    const long long active = thresholds(pf, nreg + 1, 1, false, &predicted);
+    if (active > 0) {
+      setup_excess(pf, p, 0, nreg + 1);
+      solve_state(lf_tuning{});
+      hipLaunchKernelGGL((k_pf_stats_impl<1>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)reach_, 1);
+      LF_HIP(hipGetLastError());
+    }
+    std::vector<RegionStat> tab((size_t)nreg);
+    LF_HIP(hipMemcpyAsync(tab.data(), pf_tab_, sizeof(RegionStat) * nreg, hipMemcpyDeviceToHost, s_));
+    std::vector<int> reach(active > 0 ? (size_t)N_ : 0);
+    if (active > 0) LF_HIP(hipMemcpyAsync(reach.data(), reach_, sizeof(int) * N_, hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    for (int k = 0; k < nreg; ++k) cnt_low[k] = tab[k].state == 1 ? (int64_t)tab[k].cnt1 : 0;
+    for (long long v = 0; v < N_; ++v)
+      low[v] = (reg[v] >= 0 && active > 0 && tab[reg[v]].state == 1 && reach[v]) ? 1 : 0;
+    stats.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    pf_solved = false;
+  }
   void set_base_network(const int64_t* red, const int64_t* alpha_lattice_major) override {
     tv_mode_ = false;
     // host arrays: red[N], alpha[N*K] (node-major, lattice direction order)
@@ -3150,12 +3261,15 @@ class Engine final : public MinCutEngine {
   // pass after the same launches).  deferred (capped passes): the batch that
   // reaches the cap is not checked here; *deferred tells the caller to read
   // the border counter with its own next read.
+  // timed: the device time of the launch batches (hipEvents around each
+  // batch, read once the batch is synchronised) goes to stats.bfs_device_seconds
   template <class F>
-  bool run_tiled(F&& launch, long long cap = 0, bool resume = false, bool* deferred = nullptr) {
+  bool run_tiled(F&& launch, long long cap = 0, bool resume = false, bool* deferred = nullptr, bool timed = false) {
     if (!resume) fill_many({{u32(bflags_[bcur_]), 1u, bntiles_}, {u32(bflags_[bcur_ ^ 1]), 0u, bntiles_}});
     long long launched = 0;
     for (int batch = 2;; batch = std::min(batch * 2, 8)) {
       if (cap > 0 && launched >= cap) return false;  // stopped before the fixpoint
+      if (timed) LF_HIP(hipEventRecord(evb_[0], s_));
       for (int b = 0; b < batch; ++b) {
         if (b == batch - 1) zero_counter();
         BP q = bp_;
@@ -3167,11 +3281,17 @@ class Engine final : public MinCutEngine {
         ++launched;
       }
       LF_HIP(hipGetLastError());
+      if (timed) {
+        LF_HIP(hipEventRecord(evb_[1], s_));
+        bfs_time_pending_ = true;
+      }
       if (deferred && cap > 0 && launched >= cap) {
         *deferred = true;
         return false;
       }
-      if (read_counter() == 0) return true;
+      const unsigned long long border = read_counter();
+      take_bfs_time();
+      if (border == 0) return true;
       // labels / reach bits only move one way, so the fixpoint is reached;
       // a pass that does not get there within this many launches is a bug
       if (launched > 16 * bntiles_ + 4096) fail(LF_ERR_STATE, "tiled exact pass did not converge");
@@ -3234,6 +3354,15 @@ class Engine final : public MinCutEngine {
     mbox_dirty_ = true;
   }
 
+  // add the device time of the last timed batch once the stream passed it
+  void take_bfs_time() {
+    if (!bfs_time_pending_) return;
+    float ms = 0;
+    LF_HIP(hipEventElapsedTime(&ms, evb_[0], evb_[1]));
+    stats.bfs_device_seconds += ms * 1e-3;
+    bfs_time_pending_ = false;
+  }
+
   void launch_flood(const BP& q) {
     if (bits_)
       hipLaunchKernelGGL(k_flood_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, reach_);
@@ -3278,9 +3407,11 @@ class Engine final : public MinCutEngine {
         mask_valid_ = true;
       }
       *exact = run_tiled([&](const BP& q) {
-        hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, h_);
+        BP qv = q;
+        qv.visits = bvisits_;
+        hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, qv, mask_, h_);
         ++stats.bfs_launches;
-      }, cap, resume, &deferred);
+      }, cap, resume, &deferred, true);
     }
     // push rounds start with exactly the tiles holding active nodes, and with
     // empty mailboxes (delivered above: no mail or delivery bits left)
@@ -3305,6 +3436,7 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipMemcpyAsync(host_small_, counter_ptr(), 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s_));
     LF_HIP(hipStreamSynchronize(s_));
     const unsigned long long* hs = reinterpret_cast<const unsigned long long*>(host_small_);
+    take_bfs_time();
     if (deferred) *exact = hs[0] == 0;
     tiles_seen_ = hs[1];
     return (long long)hs[4];
@@ -3350,6 +3482,9 @@ class Engine final : public MinCutEngine {
   long long* part_ = nullptr;
   void* host_small_ = nullptr;
   hipEvent_t ev_[2] = {nullptr, nullptr};
+  hipEvent_t evb_[2] = {nullptr, nullptr};  // around a batch of exact-BFS launches
+  bool bfs_time_pending_ = false;
+  unsigned long long* bvisits_ = nullptr;   // exact-BFS node visits, one slot per XCD
   unsigned long long max_red_ = 0, max_alpha_ = 0;
   const bool trace_ = std::getenv("LF_TRACE") != nullptr;
   const bool trace_pf_ = std::getenv("LF_TRACE_PF") != nullptr;

@@ -366,6 +366,8 @@ class GenericEngine final : public MinCutEngine {
   void set_tv_network(const int64_t*, const int64_t*, int) override { unsupported(); }
   void pf_values_device(int64_t*, hipStream_t) override { unsupported(); }
   void pf_cut_device(int32_t*, hipStream_t) override { unsupported(); }
+  void pf_regions_device(int32_t*, int64_t*, hipStream_t) override { unsupported(); }
+  void pf_check_regions(const int64_t*, const int32_t*, int, int64_t*, uint8_t*) override { unsupported(); }
   void set_base_network(const int64_t*, const int64_t*) override { unsupported(); }
   void batch_set_layout(const std::vector<BatchRect>&) override { unsupported(); }
   void batch_add_energies_device(int, const int64_t*, const int64_t*, hipStream_t) override { unsupported(); }

@@ -558,3 +558,59 @@ def calculate2dTVPath(X, lambdas, kernel="Star2d_16", return_stats=False):
     if lam.size:
         R = R / lam[-1]
     return (R, st) if return_stats else R
+
+
+class TVRegPath(object):
+    """The event-driven regularization path of the 2-d image X over lambda in
+    [0, lambda_max] (TVSolver::run, tv_solver.hpp:114-149,300-642): the exact
+    solution at lambda_max, then closed-form joins and device-verified splits
+    of its level sets down to lambda = 0.  Built once; ``path(lambdas)`` gives
+    the slices at any lambdas in that range (getRegularizationPath,
+    tv_solver.hpp:181-244), the same values as ``calculate2dTVPath``'s C++
+    counterpart ``_tv_path`` with lambda_max the largest lambda of the list."""
+
+    def __init__(self, X, lambda_max, kernel="Star2d_16", layers=0):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        if X.ndim != 2:
+            raise ValueError("TVRegPath needs a 2-d array")
+        self._shape = X.shape
+        self._p = None
+        h = ctypes.c_void_p()
+        check(lib().lf_tvpath_create(ctypes.byref(h), kernel.encode(), X.shape[0], X.shape[1], X.ctypes.data,
+                                     float(lambda_max), int(layers)))
+        self._p = h
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value:
+            try:
+                lib().lf_tvpath_destroy(p)
+            except Exception:
+                pass
+            self._p = None
+
+    def __call__(self, lambdas):
+        lam = np.ascontiguousarray(lambdas, dtype=np.float64).reshape(-1)
+        R = np.empty((lam.size,) + self._shape, dtype=np.float64)
+        check(lib().lf_tvpath_eval(self._p, lam.size, lam.ctypes.data, R.ctypes.data))
+        return R
+
+    def stats(self):
+        st = _lib.lf_tvpath_stats()
+        check(lib().lf_tvpath_get_stats(self._p, ctypes.byref(st)))
+        return st.as_dict()
+
+
+def calculate2dTVPathEvents(X, lambdas, kernel="Star2d_16", return_stats=False):
+    """calculate2dTV(nx, ny, f, vector<double> lambda) (tv_solver.hpp:659-681)
+    through the event-driven path: one path up to the largest lambda, read at
+    every lambda of the list.  Slices as ``_tv_path`` gives them (before the
+    Python layer's division by the last lambda)."""
+    lam = np.ascontiguousarray(lambdas, dtype=np.float64).reshape(-1)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    if lam.size == 0:
+        R = np.empty((0,) + X.shape, np.float64)
+        return (R, {}) if return_stats else R
+    path = TVRegPath(X, max(float(np.maximum(lam, 0).max()), 0.0), kernel)
+    R = path(np.maximum(lam, 0))
+    return (R, path.stats()) if return_stats else R
