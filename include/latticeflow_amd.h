@@ -224,31 +224,33 @@ int lf_tv_path_guided(const char *kernel, int64_t nx, int64_t ny, const double *
  * The whole path of one image over lambda in [0, lambda_max], as the
  * reference's TVSolver builds it (TVSolver::run, tv_solver.hpp:114-149;
  * _constructInitialRegPathsFromSolvedLattice :300-564; TVRegPathSegment,
- * tv_regpath_node.hpp:67-1028): the exact solution at lambda_max, then level
- * sets that join in closed form and split where a minimum cut says so, down
- * to lambda = 0.  Every minimum cut runs on the device; the checks of
- * overlapping level sets are batched as `layers` stacked copies of the
- * lattice (0 = 4; else 2..64).  lf_tvpath_eval then gives the path at any
- * lambdas in [0, lambda_max] (getRegularizationPath, tv_solver.hpp:181-244:
- * slice l for lambdas[l], negative lambdas count as 0, a lambda above
- * lambda_max is LF_ERR_OUT_OF_RANGE), the same output as lf_tv_path's with
- * lambda_max the largest lambda of the list. */
+ * tv_regpath_node.hpp:67-1028): the exact solution at lambda_max on the
+ * device, then level sets that join in closed form and split where a minimum
+ * cut says so, down to lambda = 0.  Every new level set is checked by one
+ * minimum cut (more where it splits): level sets of at most host_max nodes on
+ * the host, larger ones on the device (0: 32768; negative: all on the
+ * device).  lf_tvpath_eval then gives the path at any lambdas in
+ * [0, lambda_max] (getRegularizationPath, tv_solver.hpp:181-244, traced per
+ * pixel on the device): slice l for lambdas[l], negative lambdas count as 0,
+ * a lambda above lambda_max is LF_ERR_OUT_OF_RANGE.  The output equals
+ * lf_tv_path's with lambda_max the largest lambda of the list. */
 typedef struct lf_tvpath lf_tvpath;
 typedef struct lf_tvpath_stats {
-  int64_t initial_regions;   /* level sets at lambda_max */
-  int64_t segments;          /* level sets of the path, each alive on an interval */
-  int64_t joins;             /* join events */
-  int64_t splits;            /* split events */
-  int64_t checks;            /* level-set checks (one minimum cut each) */
-  int64_t device_calls;      /* batched check launches */
-  int64_t dinkelbach_steps;  /* further checks that located split points */
-  int64_t passes;            /* event passes (one more per split found) */
-  double initial_seconds;    /* the exact solution at lambda_max */
-  double check_seconds;      /* batched checks */
-  double seconds;            /* the whole construction */
+  int64_t initial_regions;      /* level sets at lambda_max */
+  int64_t segments;             /* level sets of the path, each alive on an interval */
+  int64_t joins;                /* join events */
+  int64_t splits;               /* split events */
+  int64_t checks;               /* level-set checks (one minimum cut each) */
+  int64_t host_checks;          /* of those, on the host */
+  int64_t device_calls;         /* batched device check launches */
+  int64_t dinkelbach_steps;     /* checks that located split points */
+  double initial_seconds;       /* the exact solution at lambda_max */
+  double device_check_seconds;  /* device checks */
+  double host_check_seconds;    /* host checks */
+  double seconds;               /* the whole construction */
 } lf_tvpath_stats;
 int lf_tvpath_create(lf_tvpath **out, const char *kernel, int64_t nx, int64_t ny, const double *f,
-                     double lambda_max, int layers);
+                     double lambda_max, int64_t host_max);
 int lf_tvpath_eval(const lf_tvpath *p, int64_t n_lambda, const double *lambdas, double *R);
 int lf_tvpath_get_stats(const lf_tvpath *p, lf_tvpath_stats *stats);
 int lf_tvpath_destroy(lf_tvpath *p);

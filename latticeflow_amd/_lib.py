@@ -71,10 +71,10 @@ class lf_guided_stats(ctypes.Structure):
 
 class lf_tvpath_stats(ctypes.Structure):
     _fields_ = [("initial_regions", ctypes.c_int64), ("segments", ctypes.c_int64), ("joins", ctypes.c_int64),
-                ("splits", ctypes.c_int64), ("checks", ctypes.c_int64), ("device_calls", ctypes.c_int64),
-                ("dinkelbach_steps", ctypes.c_int64), ("passes", ctypes.c_int64),
-                ("initial_seconds", ctypes.c_double), ("check_seconds", ctypes.c_double),
-                ("seconds", ctypes.c_double)]
+                ("splits", ctypes.c_int64), ("checks", ctypes.c_int64), ("host_checks", ctypes.c_int64),
+                ("device_calls", ctypes.c_int64), ("dinkelbach_steps", ctypes.c_int64),
+                ("initial_seconds", ctypes.c_double), ("device_check_seconds", ctypes.c_double),
+                ("host_check_seconds", ctypes.c_double), ("seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -131,7 +131,7 @@ _SIG = {
     "lf_tv_path": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.POINTER(lf_stats)]),
     "lf_tv_path_guided": (_I, [_CP, _I64, _I64, _P, _I64, _P, _P, ctypes.c_double, _I, _P,
                                ctypes.POINTER(lf_stats), ctypes.POINTER(lf_guided_stats), _P]),
-    "lf_tvpath_create": (_I, [ctypes.POINTER(_P), _CP, _I64, _I64, _P, ctypes.c_double, _I]),
+    "lf_tvpath_create": (_I, [ctypes.POINTER(_P), _CP, _I64, _I64, _P, ctypes.c_double, _I64]),
     "lf_tvpath_eval": (_I, [_P, _I64, _P, _P]),
     "lf_tvpath_get_stats": (_I, [_P, ctypes.POINTER(lf_tvpath_stats)]),
     "lf_tvpath_destroy": (_I, [_P]),

@@ -830,12 +830,12 @@ struct lf_tvpath {
 };
 
 int lf_tvpath_create(lf_tvpath** out, const char* kernel, int64_t nx, int64_t ny, const double* f,
-                     double lambda_max, int layers) {
+                     double lambda_max, int64_t host_max) {
   return guarded([&] {
     if (!out) fail(LF_ERR_BAD_ARG, "null argument");
     *out = nullptr;
     std::unique_ptr<lf_tvpath> p(new lf_tvpath);
-    p->path.reset(new TvPath(kernel, nx, ny, f, lambda_max, layers));
+    p->path.reset(new TvPath(kernel, nx, ny, f, lambda_max, host_max));
     *out = p.release();
   });
 }
@@ -856,11 +856,12 @@ int lf_tvpath_get_stats(const lf_tvpath* p, lf_tvpath_stats* out) {
     out->joins = s.joins;
     out->splits = s.splits;
     out->checks = s.checks;
+    out->host_checks = s.host_checks;
     out->device_calls = s.device_calls;
     out->dinkelbach_steps = s.dinkelbach_steps;
-    out->passes = s.passes;
     out->initial_seconds = s.initial_seconds;
-    out->check_seconds = s.check_seconds;
+    out->device_check_seconds = s.device_check_seconds;
+    out->host_check_seconds = s.host_check_seconds;
     out->seconds = s.seconds;
   });
 }

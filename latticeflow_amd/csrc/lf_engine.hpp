@@ -9,16 +9,9 @@
 #include <vector>
 
 #include "../../include/latticeflow_amd.h"
+#include "lf_error.hpp"
 
 namespace lfamd {
-
-// Thrown inside the library and turned into an LF_ERR_* code at the C ABI.
-struct LfError {
-  int code;
-  std::string msg;
-};
-
-[[noreturn]] inline void fail(int code, const std::string& msg) { throw LfError{code, msg}; }
 
 #define LF_HIP(call)                                                                  \
   do {                                                                                \

@@ -10,26 +10,27 @@
 namespace lfamd {
 
 struct TvPathStats {
-  int64_t initial_regions = 0;   // level sets at lambda_max
-  int64_t segments = 0;          // level sets of the path (each lives on an interval of lambda)
-  int64_t joins = 0;             // join events
-  int64_t splits = 0;            // split events
-  int64_t checks = 0;            // level-set checks (one minimum cut of one level set each)
-  int64_t device_calls = 0;      // batched check launches
-  int64_t dinkelbach_steps = 0;  // further checks to locate a split point
-  int64_t passes = 0;            // event passes (one more per split found)
-  double initial_seconds = 0;    // divide and conquer at lambda_max
-  double check_seconds = 0;      // batched checks (device and staging)
-  double seconds = 0;            // whole construction
+  int64_t initial_regions = 0;    // level sets at lambda_max
+  int64_t segments = 0;           // level sets of the path (each lives on an interval of lambda)
+  int64_t joins = 0;              // join events
+  int64_t splits = 0;             // split events
+  int64_t checks = 0;             // level-set checks (one minimum cut of one level set each)
+  int64_t host_checks = 0;        // of those, on the host
+  int64_t device_calls = 0;       // batched device check launches
+  int64_t dinkelbach_steps = 0;   // checks that located split points
+  double initial_seconds = 0;     // exact solution at lambda_max
+  double device_check_seconds = 0;
+  double host_check_seconds = 0;
+  double seconds = 0;             // whole construction
 };
 
 // The path of one image: built once, then evaluated at any lambda in
-// [0, lambda_max].  All minimum cuts run on the device.
+// [0, lambda_max].
 class TvPath {
  public:
-  // layers: overlapping level sets checked in one launch (stacked copies of
-  // the lattice); 0 picks the default.
-  TvPath(const char* kernel, int64_t nx, int64_t ny, const double* f, double lambda_max, int layers);
+  // host_max: level sets of at most this many nodes are checked on the host,
+  // larger ones on the device (0: the default; negative: all on the device).
+  TvPath(const char* kernel, int64_t nx, int64_t ny, const double* f, double lambda_max, int64_t host_max);
   ~TvPath();
   void eval(int64_t n_lambda, const double* lambdas, double* R) const;
   const TvPathStats& stats() const;

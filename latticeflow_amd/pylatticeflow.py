@@ -567,9 +567,12 @@ class TVRegPath(object):
     of its level sets down to lambda = 0.  Built once; ``path(lambdas)`` gives
     the slices at any lambdas in that range (getRegularizationPath,
     tv_solver.hpp:181-244), the same values as ``calculate2dTVPath``'s C++
-    counterpart ``_tv_path`` with lambda_max the largest lambda of the list."""
+    counterpart ``_tv_path`` with lambda_max the largest lambda of the list.
+    ``host_max``: level sets of at most this many nodes are checked on the
+    host, larger ones on the device (0: the library's default; negative: all
+    on the device)."""
 
-    def __init__(self, X, lambda_max, kernel="Star2d_16", layers=0):
+    def __init__(self, X, lambda_max, kernel="Star2d_16", host_max=0):
         X = np.ascontiguousarray(X, dtype=np.float64)
         if X.ndim != 2:
             raise ValueError("TVRegPath needs a 2-d array")
@@ -577,7 +580,7 @@ class TVRegPath(object):
         self._p = None
         h = ctypes.c_void_p()
         check(lib().lf_tvpath_create(ctypes.byref(h), kernel.encode(), X.shape[0], X.shape[1], X.ctypes.data,
-                                     float(lambda_max), int(layers)))
+                                     float(lambda_max), int(host_max)))
         self._p = h
 
     def __del__(self):

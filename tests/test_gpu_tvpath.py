@@ -23,12 +23,12 @@ def _err(R, Rr, f):
     return float(np.abs(R - Rr).max() / S)
 
 
-def _check(oracle, f, lams, kernel, tol=1e-9, layers=0):
+def _check(oracle, f, lams, kernel, tol=1e-9, host_max=0):
     from latticeflow_amd import TVRegPath
     lams = np.asarray(lams, np.float64)
     lmax = float(lams.max())
-    print("case", kernel, f.shape, "lambda_max", lmax, "layers", layers, flush=True)
-    path = TVRegPath(f, lmax, kernel, layers=layers)
+    print("case", kernel, f.shape, "lambda_max", lmax, "host_max", host_max, flush=True)
+    path = TVRegPath(f, lmax, kernel, host_max=host_max)
     R = path(lams)
     Rr = oracle.tv_path(f, lams, kernel)
     st = path.stats()
@@ -66,19 +66,33 @@ def test_reference_images(oracle):
         _check(oracle, imgs[name], lams, "Star2d_16")
 
 
-def test_layers_agree(oracle):
-    # the number of stacked copies only changes how checks are batched
+def test_host_and_device_checks_agree(oracle):
+    # the device and the host cut a level set by the same rule, so where the
+    # checks run does not change the path
     from latticeflow_amd import TVRegPath
     from latticeflow_amd import workloads as W
     f = W.tv_voronoi(seed=3, shape=(30, 34), cells=9)
     lams = np.linspace(0, 3.0, 13)
     outs = []
-    for layers in (2, 4, 9):
-        p = TVRegPath(f, 3.0, "Full2d_8", layers=layers)
+    for host_max in (-1, 40, 1 << 20):
+        p = TVRegPath(f, 3.0, "Full2d_8", host_max=host_max)
         outs.append(p(lams))
-        print(layers, p.stats(), flush=True)
+        st = p.stats()
+        print(host_max, st, flush=True)
+        if host_max < 0:
+            assert st["host_checks"] == 0 and st["device_calls"] > 0
+        if host_max == 1 << 20:
+            assert st["device_calls"] == 0
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     assert _err(outs[0], oracle.tv_path(f, lams, "Full2d_8"), f) <= 1e-9
+
+
+def test_device_checks_vs_oracle(oracle):
+    # every check on the device
+    from latticeflow_amd import workloads as W
+    f = W.tv_voronoi(seed=8, shape=(36, 28), cells=8)
+    st = _check(oracle, f, np.linspace(0, 4.0, 17), "Star2d_16", host_max=-1)
+    assert st["host_checks"] == 0
 
 
 def test_splits_random_search(oracle):
@@ -119,7 +133,5 @@ def test_errors():
         p([3.0])  # above lambda_max
     with pytest.raises(ValueError):
         TVRegPath(f, 2.0, "Full3d_6")
-    with pytest.raises(ValueError):
-        TVRegPath(f, 2.0, "Full2d_8", layers=1)
     one = TVRegPath(np.array([[0.25]]), 1.0, "Full2d_4")
     assert np.allclose(one([0.0, 0.5, 1.0]), 0.25)
