@@ -12,8 +12,9 @@ same for the exact global-relabel BFS and for the whole solve, and the
 single-thread CPU baseline (the oracle's PRFlow restatement) timed on this host.
 The line also carries the other two parts of the metric, each measured in the
 same run: "parts" holds the 256^3 26-connected min-cut (Medges/s, parity on a
-sample) and the 1024^2 x 32-lambda TV path, plain and learned (--no-parts
-leaves them out).
+sample), the 1024^2 x 32-lambda TV path, plain and learned, and the
+event-driven TV path (the whole path over lambda, read at the same 32
+lambdas) on a 128^2 image of the same kind (--no-parts leaves them out).
 """
 from __future__ import annotations
 
@@ -219,7 +220,8 @@ def main():
     }
     if a.workload == "seg8" and not shape_given and not a.no_parts:
         del lf
-        out["parts"] = {"blobs26_256x256x256": part_blobs26(a), "tvpath_1024x1024_32": part_tvpath(a)}
+        out["parts"] = {"blobs26_256x256x256": part_blobs26(a), "tvpath_1024x1024_32": part_tvpath(a),
+                        "tvpath_events_128x128": part_tvpath_events(a)}
     print(json.dumps(out), flush=True)
 
 
@@ -314,6 +316,57 @@ def part_tvpath(a):
     ns = argparse.Namespace(**vars(a))
     ns.steps, ns.warmup, ns.shape = 1, 1, ""
     return tvpath_record(ns)
+
+
+def part_tvpath_events(a, n=128, cells=16):
+    """The event-driven TV path (TVRegPath: the exact solution at lambda_max =
+    2, then joins and device-checked splits down to lambda = 0) of an n x n
+    Voronoi image (``cells`` cells + N(0, 0.1)), read at the 32 lambdas
+    log-spaced in [0.01, 2]; timed against the per-lambda layered path on the
+    same lambdas, with the oracle's parity on two of them."""
+    import numpy as np
+    import torch
+    torch.zeros(1, device="cuda")
+    from latticeflow_amd import TVRegPath, workloads as W
+    from latticeflow_amd.pylatticeflow import _tv_path
+    f = W.tv_voronoi(seed=5, shape=(n, n), cells=cells)
+    lams = W.tv_lambdas()
+    lmax = float(lams.max())
+    TVRegPath(f[:32, :32], lmax, "Full2d_8")  # warm-up of the device code paths
+    t0 = time.perf_counter()
+    path = TVRegPath(f, lmax, "Full2d_8")
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    R = path(lams)
+    t_eval = time.perf_counter() - t0
+    st = path.stats()
+    _tv_path(f, lams, "Full2d_8")
+    t0 = time.perf_counter()
+    Rl, _ = _tv_path(f, lams, "Full2d_8")
+    t_layered = time.perf_counter() - t0
+    S = 2 * float(np.abs(f - f.mean()).max())
+    out = {"workload": "tvpath events %dx%d Full2d_8 Voronoi(%d cells)+N(0,0.1), lambda_max 2, read at 32 lambdas "
+                       "log[0.01,2]" % (n, n, cells),
+           "build_seconds": round(t_build, 3), "eval_seconds": round(t_eval, 4),
+           "layered_seconds_same_lambdas": round(t_layered, 3),
+           "max_diff_vs_layered_over_scale": float(np.abs(R - Rl).max() / S),
+           "stats": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}}
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        pick = [0, 31]
+        tc = time.perf_counter()
+        Rr = O.tv_path(f, lams[pick], "Full2d_8")
+        tcpu = time.perf_counter() - tc
+        err = float(np.abs(R[pick] - Rr).max() / S)
+        out["parity_vs_cpu"] = {"max_err_over_scale": err, "within_1e-9": err <= 1e-9}
+        out["cpu_baseline"] = {"value": round(tcpu / len(pick) * len(lams), 2),
+                               "unit": "s (32 lambdas, extrapolated)", "cores": 1, "kind": "port",
+                               "sample": "lambdas %s with the oracle's per-lambda divide and conquer, %.2f s for %d "
+                                         "lambdas, scaled to 32" % ([round(float(lams[i]), 4) for i in pick], tcpu,
+                                                                    len(pick)),
+                               **host_cpu()}
+    return out
 
 
 def measured_traffic(workload, tuning, with_source=False):
