@@ -12,7 +12,8 @@ same for the exact global-relabel BFS and for the whole solve, and the
 single-thread CPU baseline (the oracle's PRFlow restatement) timed on this host.
 The line also carries the other two parts of the metric, each measured in the
 same run: "parts" holds the 256^3 26-connected min-cut (Medges/s, parity on a
-sample), the 1024^2 x 32-lambda TV path, plain and learned, and the
+sample), the same volume unthresholded (a harder 3-D min-cut), the 1024^2 x
+32-lambda TV path, plain and learned, and the
 event-driven TV path (the whole path over lambda, read at the same 32
 lambdas) on a 128^2 image of the same kind (--no-parts leaves them out).
 """
@@ -35,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # (rounds per relabel, local iterations, tile shape, relabel launches), fixed
 # so that the line and its rocprofv3 profile run the same schedule;
 # --autotune runs the search instead.
-DEFAULT_TUNING = {"seg8": (4, 4, 1, 2), "blobs26": (4, 4, 1, 2), "grid4": (0, 0, 3, 0)}
+DEFAULT_TUNING = {"seg8": (4, 4, 1, 2), "blobs26": (4, 4, 1, 2), "grid4": (0, 0, 3, 0), "noise26": (0, 0, 1, 0)}
 
 
 def make_workload(name, shape):
@@ -46,6 +47,8 @@ def make_workload(name, shape):
         return W.grid4_256(seed=1, shape=shape or (256, 256)), 4
     if name == "blobs26":
         return W.blobs26(seed=3, shape=shape or (256, 256, 256)), 8
+    if name == "noise26":
+        return W.noise26(seed=3, shape=shape or (256, 256, 256)), 8
     raise SystemExit("unknown workload " + name)
 
 
@@ -220,8 +223,8 @@ def main():
     }
     if a.workload == "seg8" and not shape_given and not a.no_parts:
         del lf
-        out["parts"] = {"blobs26_256x256x256": part_blobs26(a), "tvpath_1024x1024_32": part_tvpath(a),
-                        "tvpath_events_128x128": part_tvpath_events(a)}
+        out["parts"] = {"blobs26_256x256x256": part_blobs26(a), "noise26_256x256x256": part_noise26(a),
+                        "tvpath_1024x1024_32": part_tvpath(a), "tvpath_events_128x128": part_tvpath_events(a)}
     print(json.dumps(out), flush=True)
 
 
@@ -257,13 +260,24 @@ def sample_parity(w, cshape, cap_bytes, ref):
 def part_blobs26(a):
     """The 256^3 26-connected min-cut of the metric (blobs volume, int64), timed
     like the default line, with its CPU baseline and sample parity."""
+    return part_volume26(a, "blobs26", 2)
+
+
+def part_noise26(a):
+    """The same 256^3 volume without the threshold into blobs (soft boundaries,
+    flow travels): a harder 26-connected min-cut beside the metric's; CPU
+    baseline and parity on the leading 1/32 of the volume."""
+    return part_volume26(a, "noise26", 32)
+
+
+def part_volume26(a, name, sample_div):
     import numpy as np
     import torch
     from latticeflow_amd import LatticeFlow, workloads as W
-    w, cap_bytes = make_workload("blobs26", None)
+    w, cap_bytes = make_workload(name, None)
     shape = w["shape"]
     lf = LatticeFlow(w["kernel"], "LatticeEnergyMinimizer", shape, numerical_precision=1.0, cap_bytes=cap_bytes)
-    r, it, ts, cap = DEFAULT_TUNING["blobs26"]
+    r, it, ts, cap = DEFAULT_TUNING[name]
     lf.setTuning(rounds_per_relabel=r, local_iters=it, tile_shape=ts, relabel_launches=cap)
     du = torch.from_numpy(np.ascontiguousarray(w["unary"])).cuda()
     dp = torch.from_numpy(np.ascontiguousarray(w["pairwise"])).cuda()
@@ -285,14 +299,15 @@ def part_blobs26(a):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     achieved = tiles * st["tile_nodes"] * st["bytes_per_node"] / push_s / 1e9 if push_s > 0 else 0.0
-    res = {"workload": "blobs26 256x256x256 Star3d_26 int64", "value": round(edges / dt / 1e6, 3),
-           "unit": "Medges/s", "ms_per_step": round(dt * 1e3, 3), "steps": steps, "edges": edges,
-           "flow": lf.flow(), "k_push_frac": round(achieved / HBM_PEAK_GBS, 4)}
+    res = {"workload": "%s %s %s int64" % (name, "x".join(map(str, shape)), w["kernel"]),
+           "value": round(edges / dt / 1e6, 3), "unit": "Medges/s", "ms_per_step": round(dt * 1e3, 3),
+           "steps": steps, "edges": edges, "flow": lf.flow(), "k_push_frac": round(achieved / HBM_PEAK_GBS, 4),
+           "cycles": st["cycles"], "rounds": st["rounds"], "bfs_launches": st["bfs_launches"]}
     del lf
     if not a.no_cpu_baseline:
         from oracle import oracle as O
         O.build()
-        cshape = (shape[0] // 2,) + tuple(shape[1:])
+        cshape = (shape[0] // sample_div,) + tuple(shape[1:])
         cn = int(np.prod(cshape))
         offs = O.edge_order(w["kernel"], cshape)[0]
         red, alpha = O.reparam(cshape, offs, w["unary"][:cn], w["pairwise"][:cn])
@@ -301,7 +316,7 @@ def part_blobs26(a):
         tcpu = time.perf_counter() - tc
         res["cpu_baseline"] = {"value": round(W.n_edges(offs, cshape) / tcpu / 1e6, 3), "unit": "Medges/s",
                                "cores": 1, "kind": "port",
-                               "sample": "first-half %s, one single-thread PRFlow-restatement solve, %.2f s"
+                               "sample": "leading %s, one single-thread PRFlow-restatement solve, %.2f s"
                                          % ("x".join(map(str, cshape)), tcpu), **host_cpu()}
         res["parity_vs_cpu_sample"] = sample_parity(w, cshape, cap_bytes, ref)
     return res
@@ -381,9 +396,8 @@ def measured_traffic(workload, tuning, with_source=False):
         return none
     if not rec or any(tuning.get(k) != v for k, v in rec.get("tuning", {}).items()):
         return none
-    # the counters were taken on this kernel source only
-    src = os.path.join(ROOT, "latticeflow_amd", "csrc", "lf_mincut.hip")
-    if rec.get("kernel_source_sha256") != _sha256(src):
+    # the counters were taken on this kernel's source only
+    if rec.get("kernel_source_sha256") != push_kernel_digest():
         return none
     if with_source:
         return rec["bytes_per_launch"], ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this command on this "
@@ -391,12 +405,18 @@ def measured_traffic(workload, tuning, with_source=False):
     return rec["bytes_per_launch"]
 
 
-def _sha256(path):
+def push_kernel_digest():
+    """SHA-256 of the source that k_push is compiled from: lf_mincut.hip up to
+    the end of the kernel (its state, helpers and body; the host code after it
+    does not change what a launch reads and writes)."""
     import hashlib
+    src = os.path.join(ROOT, "latticeflow_amd", "csrc", "lf_mincut.hip")
     try:
-        return hashlib.sha256(open(path, "rb").read()).hexdigest()
+        text = open(src, "rb").read()
     except OSError:
         return None
+    end = text.find(b"// Work list of the tiles whose flag is set")
+    return hashlib.sha256(text[:end] if end > 0 else text).hexdigest()
 
 
 def copy_probe():

@@ -40,7 +40,10 @@ def build(verbose: bool = False, jobs: int = 8, incremental: bool = False) -> st
         objs.append(obj)
         if incremental and os.path.exists(obj):
             ot = os.path.getmtime(obj)
-            if ot > os.path.getmtime(os.path.join(CSRC, src)) and ot > head_t:
+            deps = [os.path.join(CSRC, src)]
+            if src.startswith("lf_mincut_wide"):
+                deps.append(os.path.join(CSRC, "lf_mincut.hip"))  # they include it
+            if ot > max(os.path.getmtime(d) for d in deps) and ot > head_t:
                 continue
         lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
         cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
