@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblatticeflow_amd.so")
-SOURCES = ["lf_mincut.hip", "lf_mincut_generic.hip", "lf_mincut_wide.hip", "lf_mincut_wide32.hip", "lf_mincut_wide48.hip", "lf_tvnet.hip",
+SOURCES = ["lf_mincut.hip", "lf_mincut_canon.hip", "lf_mincut_generic.hip", "lf_mincut_wide.hip", "lf_mincut_wide32.hip", "lf_mincut_wide48.hip", "lf_tvnet.hip",
            "lf_tv.cpp", "lf_tvpath_core.cpp", "lf_tvpath.cpp", "lf_bk.cpp", "lf_capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -41,7 +41,7 @@ def build(verbose: bool = False, jobs: int = 8, incremental: bool = False) -> st
         if incremental and os.path.exists(obj):
             ot = os.path.getmtime(obj)
             deps = [os.path.join(CSRC, src)]
-            if src.startswith("lf_mincut_wide"):
+            if src.startswith(("lf_mincut_wide", "lf_mincut_canon")):
                 deps.append(os.path.join(CSRC, "lf_mincut.hip"))  # they include it
             if ot > max(os.path.getmtime(d) for d in deps) and ot > head_t:
                 continue

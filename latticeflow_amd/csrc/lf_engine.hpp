@@ -112,6 +112,10 @@ std::unique_ptr<MinCutEngine> make_wide_engine(const LatticeGeom& g, int cap_byt
 std::unique_ptr<MinCutEngine> make_wide32_engine(const LatticeGeom& g, int cap_bytes);
 // ... and of Full2d_48 and Full3d_32 (lf_mincut_wide48.hip).
 std::unique_ptr<MinCutEngine> make_wide48_engine(const LatticeGeom& g, int cap_bytes);
+// Engines of the radius-1 neighbourhoods (2-D 4/8, 3-D 6/26 directions) whose
+// direction table is the canonical lattice order, compiled into the kernels
+// (lf_mincut_canon.hip); nullptr for any other kernel or order.
+std::unique_ptr<MinCutEngine> make_canon_engine(const LatticeGeom& g, int cap_bytes);
 
 // Learned TV predictor (lf_tvnet.hip): parameter count, workspace size,
 // forward pass, loss gradients and one Adam update, all on device pointers.

@@ -63,6 +63,57 @@ struct DirTab {
   int8_t dz[MAXK], dy[MAXK], dx[MAXK];
 };
 
+// Direction tables of the hot kernels.  RtDir reads the engine's table (a
+// kernel argument).  CDir<K, ND> is the lattice edge order of the radius-1
+// neighbourhoods (2-D: 4 and 8 directions, 3-D: 6 and 26) known at compile
+// time: the lexicographically positive offsets (dz, dy, dx) in ascending order,
+// then their negations in the same order.  With the unrolled direction loops
+// every offset, LDS address and tile-border test becomes a constant.
+struct RtDir {
+  __device__ static int z(const DirTab& t, int i) { return t.dz[i]; }
+  __device__ static int y(const DirTab& t, int i) { return t.dy[i]; }
+  __device__ static int x(const DirTab& t, int i) { return t.dx[i]; }
+};
+
+template <int K, int ND>
+__host__ __device__ constexpr bool cdir_member(int l1) {
+  return ND == 2 ? (K == 8 || (K == 4 && l1 == 1)) : (K == 26 || (K == 6 && l1 == 1));
+}
+
+template <int K, int ND>
+__host__ __device__ constexpr int cdir(int i, int axis) {
+  const int e = i < K / 2 ? i : i - K / 2;
+  const int sg = i < K / 2 ? 1 : -1;
+  int c = 0;
+  for (int z = (ND == 3 ? -1 : 0); z <= (ND == 3 ? 1 : 0); ++z)
+    for (int y = -1; y <= 1; ++y)
+      for (int x = -1; x <= 1; ++x) {
+        if (!(z > 0 || (z == 0 && (y > 0 || (y == 0 && x > 0))))) continue;
+        const int l1 = (z < 0 ? -z : z) + (y < 0 ? -y : y) + (x < 0 ? -x : x);
+        if (!cdir_member<K, ND>(l1)) continue;
+        if (c == e) return sg * (axis == 0 ? z : (axis == 1 ? y : x));
+        ++c;
+      }
+  return 0;
+}
+
+template <int K, int ND>
+struct CDir {
+  __device__ static constexpr int z(const DirTab&, int i) { return cdir<K, ND>(i, 0); }
+  __device__ static constexpr int y(const DirTab&, int i) { return cdir<K, ND>(i, 1); }
+  __device__ static constexpr int x(const DirTab&, int i) { return cdir<K, ND>(i, 2); }
+  // true when the engine's table is this order
+  static bool matches(const int* dz, const int* dy, const int* dx, int k) {
+    if (k != K) return false;
+    for (int i = 0; i < K; ++i)
+      if (dz[i] != cdir<K, ND>(i, 0) || dy[i] != cdir<K, ND>(i, 1) || dx[i] != cdir<K, ND>(i, 2)) return false;
+    return true;
+  }
+};
+static_assert(cdir<8, 2>(0, 2) == 1 && cdir<8, 2>(1, 1) == 1 && cdir<8, 2>(1, 2) == -1 && cdir<8, 2>(4, 2) == -1,
+              "2-D 8-neighbourhood order");
+static_assert(cdir<26, 3>(4, 0) == 1 && cdir<26, 3>(4, 1) == -1 && cdir<26, 3>(25, 0) == -1, "3-D 26 order");
+
 template <int TD_, int TH_, int TW_, int RZ_, int RY_, int RX_>
 struct Geo {
   static constexpr int TD = TD_, TH = TH_, TW = TW_, RZ = RZ_, RY = RY_, RX = RX_;
@@ -193,7 +244,7 @@ __device__ inline void tile_coords(int t, int& z, int& y, int& x) {
 // One push / relabel round over the tiles of the work list; a fixed grid of
 // resident blocks walks the list, so a round over few active tiles costs
 // little more than those tiles.
-template <int K, class G, typename T>
+template <int K, class G, typename T, class DIR>
 __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
   constexpr int NT = G::NT;
   __shared__ T s_res[K * NT];
@@ -274,7 +325,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
           const int gz = z0 + z, gy = y0 + y, gx = x0 + x;
 #pragma unroll
           for (int i = 0; i < K; ++i) {
-            const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+            const int dz = DIR::z(p.dirs, i), dy = DIR::y(p.dirs, i), dx = DIR::x(p.dirs, i);
             const int lz = z + dz, ly = y + dy, lx = x + dx;
             const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
             const int side = (lz < 0 ? 0 : (lz >= G::TD ? 2 : 1)) * 9 + (ly < 0 ? 0 : (ly >= G::TH ? 2 : 1)) * 3 +
@@ -327,7 +378,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
 #pragma unroll
           for (int u = 0; u < HOIST && c + u < K; ++u) {
             const int i = c + u;
-            const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+            const int dz = DIR::z(p.dirs, i), dy = DIR::y(p.dirs, i), dx = DIR::x(p.dirs, i);
             const int lz = z + dz, ly = y + dy, lx = x + dx;
             const int nz = gz + dz, ny = gy + dy, nx = gx + dx;
             const int side = (lz < 0 ? 0 : (lz >= G::TD ? 2 : 1)) * 9 + (ly < 0 ? 0 : (ly >= G::TH ? 2 : 1)) * 3 +
@@ -400,13 +451,13 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
 #pragma unroll
           for (int u = 0; u < HOIST && c + u < K; ++u) {
             const int i = c + u;
-            hn[u] = s_h[hi + (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i]];
+            hn[u] = s_h[hi + (DIR::z(p.dirs, i) * G::HH + DIR::y(p.dirs, i)) * G::HW + DIR::x(p.dirs, i)];
             rr[u] = s_res[i * NT + t];
           }
 #pragma unroll
           for (int u = 0; u < HOIST && c + u < K; ++u) {
             const int i = c + u;
-            const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+            const int dz = DIR::z(p.dirs, i), dy = DIR::y(p.dirs, i), dx = DIR::x(p.dirs, i);
             if (e > 0 && hn[u] < hv) {
               const T r = rr[u];
               if (r > 0) {
@@ -450,7 +501,7 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
 #pragma unroll
           for (int u = 0; u < HOIST && c + u < K; ++u) {
             const int i = c + u;
-            hn[u] = s_h[hi + (p.dirs.dz[i] * G::HH + p.dirs.dy[i]) * G::HW + p.dirs.dx[i]];
+            hn[u] = s_h[hi + (DIR::z(p.dirs, i) * G::HH + DIR::y(p.dirs, i)) * G::HW + DIR::x(p.dirs, i)];
             rr[u] = s_res[i * NT + t];
           }
 #pragma unroll
@@ -539,7 +590,7 @@ __global__ __launch_bounds__(NTHREADS) void k_flags_to_list(const int* __restric
 
 // ---- mailbox delivery (before any exact pass) --------------------------------
 
-template <int K, class G, typename T>
+template <int K, class G, typename T, class DIR>
 __device__ inline void deliver_node(const KP<T>& p, long long v) {
   const int x = (int)(v % p.W), y = (int)((v / p.W) % p.H), z = (int)(v / ((long long)p.W * p.H));
   const int lx = x % G::TW, ly = y % G::TH, lz = z % G::TD;
@@ -555,7 +606,7 @@ __device__ inline void deliver_node(const KP<T>& p, long long v) {
   T a[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) {
-    const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+    const int dz = DIR::z(p.dirs, i), dy = DIR::y(p.dirs, i), dx = DIR::x(p.dirs, i);
     const int nz = z + dz, ny = y + dy, nx = x + dx;
     const int qz = lz + dz, qy = ly + dy, qx = lx + dx;
     const int side = (qz < 0 ? 0 : (qz >= G::TD ? 2 : 1)) * 9 + (qy < 0 ? 0 : (qy >= G::TH ? 2 : 1)) * 3 +
@@ -585,11 +636,11 @@ __device__ inline void deliver_node(const KP<T>& p, long long v) {
 // Delivery pass; with init_labels it also sets the BFS start labels (h = 0 on
 // sink-capable nodes, INF_H elsewhere) from the delivered excess, which saves
 // the separate label pass when the arc masks are current.
-template <int K, class G, typename T>
+template <int K, class G, typename T, class DIR>
 __global__ __launch_bounds__(NTHREADS) void k_deliver(KP<T> p, int init_labels) {
   const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
   if (v >= p.N) return;
-  deliver_node<K, G, T>(p, v);
+  deliver_node<K, G, T, DIR>(p, v);
   if (init_labels) p.h[v] = p.ex[v] < 0 ? 0 : INF_H;
 }
 
@@ -677,7 +728,7 @@ __device__ inline void flag_around(const BP& p, int tile) {
 // relax their in-neighbours inside the tile, until no label changes.  Labels
 // only decrease and every value is the length of a real residual path, so the
 // fixpoint over all tiles is the exact BFS distance.
-template <int K, class B, typename MT>
+template <int K, class B, typename MT, class DIR>
 __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restrict__ mask, int* __restrict__ hgl) {
   // a list holds up to half the tile; a step whose list overflowed walks the
   // listed-node bits instead (every claimed node has its bit set), so the
@@ -754,7 +805,7 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
 #pragma unroll
     for (int i = 0; i < K; ++i)
       if (m & ((MT)1 << i)) {
-        const int dn = s_d[hi + (p.dirs.dz[i] * B::HH + p.dirs.dy[i]) * B::HW + p.dirs.dx[i]] + 1;
+        const int dn = s_d[hi + (DIR::z(p.dirs, i) * B::HH + DIR::y(p.dirs, i)) * B::HW + DIR::x(p.dirs, i)] + 1;
         best = dn < best ? dn : best;
       }
     if (best < dv) {
@@ -797,7 +848,7 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
       const int z = v / (B::TH * B::TW), y = (v / B::TW) % B::TH, x = v % B::TW;
 #pragma unroll
       for (int j = 0; j < K; ++j) {
-        const int dz = p.dirs.dz[j], dy = p.dirs.dy[j], dx = p.dirs.dx[j];
+        const int dz = DIR::z(p.dirs, j), dy = DIR::y(p.dirs, j), dx = DIR::x(p.dirs, j);
         const int uz = z + dz, uy = y + dy, ux = x + dx;
         if (uz < 0 || uz >= B::TD || uy < 0 || uy >= B::TH || ux < 0 || ux >= B::TW) continue;
         const int u = v + (dz * B::TH + dy) * B::TW + dx;
@@ -856,7 +907,7 @@ __global__ __launch_bounds__(NTHREADS) void k_flood_init(KP<T> p, int* __restric
   reach[v] = p.ex[v] > 0 ? 1 : 0;
 }
 
-template <int K, class B, typename MT>
+template <int K, class B, typename MT, class DIR>
 __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restrict__ mask, int* __restrict__ reach) {
   __shared__ __align__(4) unsigned char s_r[(B::NH + 3) & ~3];
   __shared__ MT s_m[B::NH];
@@ -915,7 +966,7 @@ __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restr
     if (r0[j] || z >= ld || y >= lh || x >= lw) continue;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      const int q = hi + (p.dirs.dz[i] * B::HH + p.dirs.dy[i]) * B::HW + p.dirs.dx[i];
+      const int q = hi + (DIR::z(p.dirs, i) * B::HH + DIR::y(p.dirs, i)) * B::HW + DIR::x(p.dirs, i);
       if (s_r[q] && (s_m[q] & ((MT)1 << ((i + K / 2) % K)))) {
         s_r[hi] = 1;
         s_f[0][atomicAdd(&s_n[0], 1)] = (unsigned short)t;
@@ -947,7 +998,7 @@ __global__ __launch_bounds__(BTHREADS) void k_flood_tile(BP p, const MT* __restr
 #pragma unroll
       for (int i = 0; i < K; ++i) {
         if (!(m & ((MT)1 << i))) continue;
-        const int dz = p.dirs.dz[i], dy = p.dirs.dy[i], dx = p.dirs.dx[i];
+        const int dz = DIR::z(p.dirs, i), dy = DIR::y(p.dirs, i), dx = DIR::x(p.dirs, i);
         const int wz = z + dz, wy = y + dy, wx = x + dx;
         if (wz < 0 || wz >= B::TD || wy < 0 || wy >= B::TH || wx < 0 || wx >= B::TW) continue;
         const int hw = hv + (dz * B::HH + dy) * B::HW + dx;
@@ -2302,7 +2353,7 @@ TileGrid make_grid(const LatticeGeom& geo) {
 }
 
 // G0..G2: the push-tile shapes the tuner chooses from (same halo radius).
-template <int K, class G0, class G1, class G2, class B, typename T, typename MT>
+template <int K, class G0, class G1, class G2, class B, typename T, typename MT, class DIR>
 class Engine final : public MinCutEngine {
  public:
   Engine(const LatticeGeom& geo) : g_(geo) {
@@ -2350,9 +2401,9 @@ class Engine final : public MinCutEngine {
       int dev = 0, cus = 0;
       LF_HIP(hipGetDevice(&dev));
       LF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-      const void* fn[3] = {reinterpret_cast<const void*>(&k_push<K, G0, T>),
-                           reinterpret_cast<const void*>(&k_push<K, G1, T>),
-                           reinterpret_cast<const void*>(&k_push<K, G2, T>)};
+      const void* fn[3] = {reinterpret_cast<const void*>(&k_push<K, G0, T, DIR>),
+                           reinterpret_cast<const void*>(&k_push<K, G1, T, DIR>),
+                           reinterpret_cast<const void*>(&k_push<K, G2, T, DIR>)};
       for (int v = 0; v < 3; ++v) {
         int nb = 0;
         LF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn[v], NTHREADS, 0));
@@ -3316,7 +3367,7 @@ class Engine final : public MinCutEngine {
     q.wl_cnt_zero = wl_cnt_ + (wl_k_ + 2) % 3;
     with_geo([&](auto g) {
       using GG = decltype(g);
-      hipLaunchKernelGGL((k_push<K, GG, T>), dim3(wl_grid_[variant_]), dim3(NTHREADS), 0, s_, q);
+      hipLaunchKernelGGL((k_push<K, GG, T, DIR>), dim3(wl_grid_[variant_]), dim3(NTHREADS), 0, s_, q);
     });
     cur_ ^= 1;
     wl_k_ = (wl_k_ + 1) % 3;
@@ -3372,7 +3423,7 @@ class Engine final : public MinCutEngine {
     if (bits_)
       hipLaunchKernelGGL(k_flood_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, reach_);
     else
-      hipLaunchKernelGGL((k_flood_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, reach_);
+      hipLaunchKernelGGL((k_flood_tile<K, B, MT, DIR>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, q, mask_, reach_);
   }
 
   // Deliver mailboxes, BFS labels, flag the tiles of active nodes; returns the
@@ -3389,7 +3440,7 @@ class Engine final : public MinCutEngine {
     if (pushed)
       with_geo([&](auto g) {
         using GG = decltype(g);
-        hipLaunchKernelGGL((k_deliver<K, GG, T>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p,
+        hipLaunchKernelGGL((k_deliver<K, GG, T, DIR>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p,
                            fused_init ? 1 : 0);
       });
     if (pushed) mbox_dirty_ = false;  // every slot the rounds filled is delivered and cleared
@@ -3414,7 +3465,7 @@ class Engine final : public MinCutEngine {
       *exact = run_tiled([&](const BP& q) {
         BP qv = q;
         qv.visits = bvisits_;
-        hipLaunchKernelGGL((k_bfs_tile<K, B, MT>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, qv, mask_, h_);
+        hipLaunchKernelGGL((k_bfs_tile<K, B, MT, DIR>), dim3(bp_.nblocks), dim3(BTHREADS), 0, s_, qv, mask_, h_);
         ++stats.bfs_launches;
       }, cap, resume, &deferred, true);
     }
@@ -3535,11 +3586,35 @@ class Engine final : public MinCutEngine {
   long long b_nblocks_ = 0;
 };
 
-template <int K, class G0, class G1, class G2, class B>
+template <int K, class G0, class G1, class G2, class B, class DIR = RtDir>
 std::unique_ptr<MinCutEngine> make_k(const LatticeGeom& g, int cap_bytes) {
   using MT = MaskT<K>;
-  if (cap_bytes == 4) return std::unique_ptr<MinCutEngine>(new Engine<K, G0, G1, G2, B, int, MT>(g));
-  return std::unique_ptr<MinCutEngine>(new Engine<K, G0, G1, G2, B, long long, MT>(g));
+  if (cap_bytes == 4) return std::unique_ptr<MinCutEngine>(new Engine<K, G0, G1, G2, B, int, MT, DIR>(g));
+  return std::unique_ptr<MinCutEngine>(new Engine<K, G0, G1, G2, B, long long, MT, DIR>(g));
+}
+
+// Push tilings (G0..G2) and exact-pass tile (B) of the radius-1 neighbourhoods.
+struct Tiles2d1 {
+  using G0 = Geo<1, 16, 32, 0, 1, 1>;
+  using G1 = Geo<1, 8, 64, 0, 1, 1>;
+  using G2 = Geo<1, 32, 32, 0, 1, 1>;
+  using B = BGeo<1, 64, 64, 0, 1, 1>;
+};
+struct Tiles3d6 {
+  using G0 = Geo<4, 8, 32, 1, 1, 1>;
+  using G1 = Geo<2, 8, 32, 1, 1, 1>;
+  using G2 = Geo<4, 16, 16, 1, 1, 1>;
+  using B = BGeo<16, 16, 16, 1, 1, 1>;
+};
+struct Tiles3d26 {  // also the 18-neighbourhood
+  using G0 = Geo<4, 4, 16, 1, 1, 1>;
+  using G1 = Geo<2, 8, 16, 1, 1, 1>;
+  using G2 = Geo<4, 8, 8, 1, 1, 1>;
+  using B = BGeo<16, 16, 16, 1, 1, 1>;
+};
+template <int K, class TL, class DIR = RtDir>
+std::unique_ptr<MinCutEngine> make_tiled(const LatticeGeom& g, int cap_bytes) {
+  return make_k<K, typename TL::G0, typename TL::G1, typename TL::G2, typename TL::B, DIR>(g, cap_bytes);
 }
 
 int max_abs(const std::vector<int>& v) {
@@ -3556,14 +3631,13 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
   if (g.ndim == 4) return make_generic_engine(g, cap_bytes);
   const int rz = max_abs(g.dz), ry = max_abs(g.dy), rx = max_abs(g.dx);
   const bool is3d = g.ndim == 3;
+  // the lattice orders of the common radius-1 neighbourhoods have engines
+  // with the direction table compiled in (lf_mincut_canon.hip)
+  if (auto e = make_canon_engine(g, cap_bytes)) return e;
   if (!is3d && rz == 0 && ry <= 1 && rx <= 1) {
-    using G0 = Geo<1, 16, 32, 0, 1, 1>;
-    using G1 = Geo<1, 8, 64, 0, 1, 1>;
-    using G2 = Geo<1, 32, 32, 0, 1, 1>;
-    using B = BGeo<1, 64, 64, 0, 1, 1>;
-    if (g.k == 2) return make_k<2, G0, G1, G2, B>(g, cap_bytes);
-    if (g.k == 4) return make_k<4, G0, G1, G2, B>(g, cap_bytes);
-    if (g.k == 8) return make_k<8, G0, G1, G2, B>(g, cap_bytes);
+    if (g.k == 2) return make_tiled<2, Tiles2d1>(g, cap_bytes);
+    if (g.k == 4) return make_tiled<4, Tiles2d1>(g, cap_bytes);
+    if (g.k == 8) return make_tiled<8, Tiles2d1>(g, cap_bytes);
   }
   if (!is3d && rz == 0 && ry <= 2 && rx <= 2) {
     using G0 = Geo<1, 16, 32, 0, 2, 2>;
@@ -3574,10 +3648,9 @@ std::unique_ptr<MinCutEngine> make_mincut_engine(const LatticeGeom& g, int cap_b
     if (g.k == 16) return make_k<16, G0, G1, G2, B>(g, cap_bytes);
   }
   if (is3d && rz <= 1 && ry <= 1 && rx <= 1) {
-    using B = BGeo<16, 16, 16, 1, 1, 1>;
-    if (g.k == 6) return make_k<6, Geo<4, 8, 32, 1, 1, 1>, Geo<2, 8, 32, 1, 1, 1>, Geo<4, 16, 16, 1, 1, 1>, B>(g, cap_bytes);
-    if (g.k == 18) return make_k<18, Geo<4, 4, 16, 1, 1, 1>, Geo<2, 8, 16, 1, 1, 1>, Geo<4, 8, 8, 1, 1, 1>, B>(g, cap_bytes);
-    if (g.k == 26) return make_k<26, Geo<4, 4, 16, 1, 1, 1>, Geo<2, 8, 16, 1, 1, 1>, Geo<4, 8, 8, 1, 1, 1>, B>(g, cap_bytes);
+    if (g.k == 6) return make_tiled<6, Tiles3d6>(g, cap_bytes);
+    if (g.k == 18) return make_tiled<18, Tiles3d26>(g, cap_bytes);
+    if (g.k == 26) return make_tiled<26, Tiles3d26>(g, cap_bytes);
   }
   if (auto e = make_wide_engine(g, cap_bytes)) return e;
   fail(LF_ERR_UNSUPPORTED, "no device path for this kernel (more than 48 directions or radius above 4)");
