@@ -135,11 +135,20 @@ struct KP {
   T* res;
   T* ex;
   int* h;
+  // Mailboxes, one per round parity.  mbox[i][v]: flow that node v pushed along
+  // direction i into a node of another tile in a round, not yet applied there.
+  // A round adds what its tiles send to mbox (no other round writes it
+  // meanwhile) and drains mbox_in, which the round before filled and nobody
+  // writes during this round, with plain loads and stores: every tile that
+  // received flow is on the next round's list, so after a round the only
+  // undelivered flow is in the mailbox that round filled.
   T* mbox;
+  T* mbox_in;
   // mail[t] bit 9 (sz+1) + 3 (sy+1) + (sx+1): the tile at offset (sz,sy,sx)
-  // from t put flow into t's mailbox slots (a hint: the exact passes drain
-  // every mailbox anyway)
+  // from t put flow into t's mailbox slots in this round (mail_in: in the
+  // round before, read and cleared by t's block)
   unsigned* mail;
+  unsigned* mail_in;
   // dmail[t]: the same side bits, kept until the delivery pass, which reads
   // only the slots of those sides and then clears them
   unsigned* dmail;
@@ -175,9 +184,42 @@ __device__ inline void gadd(int* a, int v) { atomicAdd(a, v); }
 __device__ inline void gadd(long long* a, long long v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(a), static_cast<unsigned long long>(v));
 }
-__device__ inline int gtake(int* a) { return atomicExch(a, 0); }
-__device__ inline long long gtake(long long* a) {
-  return static_cast<long long>(atomicExch(reinterpret_cast<unsigned long long*>(a), 0ull));
+
+// Buffer addressing: the base of an array (a residual plane, or a plane
+// shifted by a neighbour offset) lives in scalar registers as a buffer
+// resource, and a node's 32-bit byte offset in one vector register serves
+// every plane.  With 64-bit flat addresses every load in flight held its own
+// register pair.  Engines using it need N * sizeof(element) < 2^32.
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+  // stride 0, the whole 32-bit offset range, dword data format (gfx9 word 3)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000);
+}
+template <typename V>
+__device__ inline V bld(const void* base, unsigned off) {
+  static_assert(sizeof(V) == 1 || sizeof(V) == 2 || sizeof(V) == 4 || sizeof(V) == 8, "element size");
+  if constexpr (sizeof(V) == 1) {
+    return (V)__builtin_amdgcn_raw_buffer_load_b8(buf_rsrc(base), (int)off, 0, 0);
+  } else if constexpr (sizeof(V) == 2) {
+    return (V)__builtin_amdgcn_raw_buffer_load_b16(buf_rsrc(base), (int)off, 0, 0);
+  } else if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(base), (int)off, 0, 0));
+  } else {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(buf_rsrc(base), (int)off, 0, 0));
+  }
+}
+template <typename V>
+__device__ inline void bst(void* base, unsigned off, V v) {
+  static_assert(sizeof(V) == 1 || sizeof(V) == 2 || sizeof(V) == 4 || sizeof(V) == 8, "element size");
+  if constexpr (sizeof(V) == 1) {
+    __builtin_amdgcn_raw_buffer_store_b8((unsigned char)v, buf_rsrc(base), (int)off, 0, 0);
+  } else if constexpr (sizeof(V) == 2) {
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, buf_rsrc(base), (int)off, 0, 0);
+  } else if constexpr (sizeof(V) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), buf_rsrc(base), (int)off, 0, 0);
+  } else {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), buf_rsrc(base), (int)off, 0, 0);
+  }
 }
 
 // Streaming kernels that reduce run a fixed grid of RED_BLOCKS blocks with a
@@ -258,9 +300,10 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
 
   if (blockIdx.x == 0 && threadIdx.x == 0) *p.wl_cnt_zero = 0;
   const int ntodo = (int)*p.wl_cnt_in;
-  // thread 0 loads the id of its block's next tile while the current one is
-  // processed
+  // thread 0 loads the id of its block's next tile, and then that tile's mail
+  // bits (final since the round before), while the current one is processed
   int next_tile = (threadIdx.x == 0 && (int)blockIdx.x < ntodo) ? p.wl_in[blockIdx.x] : -1;
+  unsigned next_mail = next_tile >= 0 ? p.mail_in[next_tile] : 0u;
   unsigned long long ntiles_done = 0;
   for (int li = blockIdx.x; li < ntodo; li += gridDim.x) {
     if (threadIdx.x == 0) {
@@ -271,7 +314,8 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
       s_cnt[0] = 0;
       s_cnt[1] = 0;
       s_nbr = 0;
-      s_mail = atomicExch(&p.mail[t], 0u);
+      s_mail = next_mail;
+      if (next_mail) p.mail_in[t] = 0u;
       ++ntiles_done;
     }
     __syncthreads();
@@ -285,10 +329,12 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
 
     // stage residuals and excess of the tile.  Slots past the lattice edge
     // load node 0 and discard it: no load waits behind a branch, so a
-    // thread's whole batch of loads is in flight at once.
+    // thread's whole batch of loads is in flight at once.  Every plane is
+    // addressed by the node's byte offset (buffer loads).
     {
       T r[G::NPT][K + 1];
       bool in[G::NPT];
+      unsigned off[G::NPT];
       // labels of the tile and its halo, loaded in the same batch
       constexpr int NHI = (G::NH + NTHREADS - 1) / NTHREADS;
       int hv[NHI];
@@ -299,22 +345,21 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         const int hz = q / (G::HH * G::HW), hy = (q / G::HW) % G::HH, hx = q % G::HW;
         const int gz = z0 + hz - G::RZ, gy = y0 + hy - G::RY, gx = x0 + hx - G::RX;
         hin[k] = q < G::NH && gz >= 0 && gz < p.D && gy >= 0 && gy < p.H && gx >= 0 && gx < p.W;
-        hv[k] = p.h[hin[k] ? ((long long)gz * p.H + gy) * p.W + gx : 0];
+        hv[k] = bld<int>(p.h, hin[k] ? (unsigned)(((long long)gz * p.H + gy) * p.W + gx) * 4u : 0u);
       }
 #pragma unroll
       for (int j = 0; j < G::NPT; ++j) {
         int z, y, x;
         tile_coords<G>(threadIdx.x + j * NTHREADS, z, y, x);
         in[j] = z < ld && y < lh && x < lw;
-        const long long g = in[j] ? ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x) : 0;
+        const unsigned g = in[j] ? (unsigned)(((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x)) : 0u;
+        off[j] = g * (unsigned)sizeof(T);
 #pragma unroll
-        for (int i = 0; i < K; ++i) r[j][i] = p.res[i * N + g];
-        r[j][K] = p.ex[g];
+        for (int i = 0; i < K; ++i) r[j][i] = bld<T>(p.res + (long long)i * N, off[j]);
+        r[j][K] = bld<T>(p.ex, off[j]);
       }
-      // kernels of at most HOIST directions take their mailbox slots (the
-      // sides the mail bits name, with atomic exchanges: a sender of this
-      // round may still add to a slot; what it adds later waits for the next
-      // round) in the same batch as the staging loads
+      // kernels of at most HOIST directions read their mailbox slots (the
+      // sides the mail bits name) in the same batch as the staging loads
       T a[G::NPT][K <= HOIST ? K : 1];
       if constexpr (K <= HOIST) {
         const unsigned mail = s_mail;
@@ -331,9 +376,12 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
             const int side = (lz < 0 ? 0 : (lz >= G::TD ? 2 : 1)) * 9 + (ly < 0 ? 0 : (ly >= G::TH ? 2 : 1)) * 3 +
                              (lx < 0 ? 0 : (lx >= G::TW ? 2 : 1));
             a[j][i] = 0;
+            // the neighbour's slot of the reverse direction: the plane shifted
+            // by the neighbour offset, at this node's byte offset
             if (side != 13 && ((mail >> side) & 1u) && in[j] && nz >= 0 && nz < p.D && ny >= 0 && ny < p.H &&
                 nx >= 0 && nx < p.W)
-              a[j][i] = gtake(&p.mbox[(long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx]);
+              a[j][i] = bld<T>(p.mbox_in + (long long)((i + K / 2) % K) * N + ((long long)dz * p.H + dy) * p.W + dx,
+                               off[j]);
           }
         }
       }
@@ -352,16 +400,20 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
           if constexpr (K <= HOIST) {
             v += a[j][i];
             add += a[j][i];
+            // a slot read with flow is cleared for the round after next
+            if (a[j][i] != 0)
+              bst<T>(p.mbox_in + (long long)((i + K / 2) % K) * N +
+                         ((long long)DIR::z(p.dirs, i) * p.H + DIR::y(p.dirs, i)) * p.W + DIR::x(p.dirs, i),
+                     off[j], (T)0);
           }
           s_res[i * NT + t] = v;
         }
         s_ex[t] = (in[j] ? r[j][K] : (T)0) + add;
       }
     }
-    // collect the flow that neighbouring tiles pushed into this tile, from the
-    // sides the mail bits name: every slot is taken with an atomic exchange
-    // (a sender of this round may still add to it; what it adds later waits
-    // for the next round), all exchanges of a thread in flight at once
+    // collect the flow that neighbouring tiles pushed into this tile in the
+    // round before, from the sides the mail bits name, all loads of a thread
+    // in flight at once; slots read with flow are cleared
     if (K > HOIST && s_mail) {
       const unsigned mail = s_mail;
       T add_e[G::NPT];
@@ -384,20 +436,27 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
             const int side = (lz < 0 ? 0 : (lz >= G::TD ? 2 : 1)) * 9 + (ly < 0 ? 0 : (ly >= G::TH ? 2 : 1)) * 3 +
                              (lx < 0 ? 0 : (lx >= G::TW ? 2 : 1));
             a[j][u] = 0;
+            const unsigned o = (unsigned)(((long long)gz * p.H + gy) * p.W + gx) * (unsigned)sizeof(T);
             if (side != 13 && ((mail >> side) & 1u) && z < ld && y < lh && x < lw && nz >= 0 && nz < p.D &&
                 ny >= 0 && ny < p.H && nx >= 0 && nx < p.W)
-              a[j][u] = gtake(&p.mbox[(long long)((i + K / 2) % K) * N + ((long long)nz * p.H + ny) * p.W + nx]);
+              a[j][u] = bld<T>(p.mbox_in + (long long)((i + K / 2) % K) * N + ((long long)dz * p.H + dy) * p.W + dx, o);
           }
         }
 #pragma unroll
         for (int j = 0; j < G::NPT; ++j) {
           const int t = threadIdx.x + j * NTHREADS;
+          int z, y, x;
+          tile_coords<G>(t, z, y, x);
 #pragma unroll
           for (int u = 0; u < HOIST && c + u < K; ++u) {
             if (a[j][u] == 0) continue;
             const int i = c + u;
             add_e[j] += a[j][u];
             s_res[i * NT + t] += a[j][u];  // only this thread touches node t's planes while staging
+            const unsigned o = (unsigned)(((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x)) * (unsigned)sizeof(T);
+            bst<T>(p.mbox_in + (long long)((i + K / 2) % K) * N +
+                       ((long long)DIR::z(p.dirs, i) * p.H + DIR::y(p.dirs, i)) * p.W + DIR::x(p.dirs, i),
+                   o, (T)0);
           }
         }
       }
@@ -405,9 +464,17 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
       for (int j = 0; j < G::NPT; ++j) s_ex[threadIdx.x + j * NTHREADS] += add_e[j];
     }
     __syncthreads();
+    if (threadIdx.x == 0 && next_tile >= 0) next_mail = p.mail_in[next_tile];
 
+    // Local iterations: relabel + list, then push.  The pass over the owned
+    // nodes relabels every node with excess and no downhill residual arc
+    // (residuals frozen: the push phase before it ended at a barrier; labels
+    // only grow, so concurrent reads of a neighbour's label are safe) and lists
+    // the nodes that are then active; the push phase walks the list with
+    // labels frozen.  Two barriers per iteration.  Nodes left with excess after
+    // the last push phase are relabelled by the first pass of their tile's next
+    // round (or by the next global relabel).
     for (int it = 0; it < p.iters; ++it) {
-      // list the active nodes (ex > 0, label < INF) with one LDS atomic per wave
       int* cnt = &s_cnt[it & 1];
 #pragma unroll
       for (int j = 0; j < G::NPT; ++j) {
@@ -416,7 +483,29 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         int z, y, x;
         tile_coords<G>(t, z, y, x);
         const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
-        const bool act = e > 0 && s_h[hi] < INF_H;
+        int hv = s_h[hi];
+        if (e > 0 && hv < INF_H) {
+          int m = INF_H;
+#pragma unroll
+          for (int c = 0; c < K; c += HOIST) {
+            int hn[HOIST];
+            T rr[HOIST];
+#pragma unroll
+            for (int u = 0; u < HOIST && c + u < K; ++u) {
+              const int i = c + u;
+              hn[u] = s_h[hi + (DIR::z(p.dirs, i) * G::HH + DIR::y(p.dirs, i)) * G::HW + DIR::x(p.dirs, i)];
+              rr[u] = s_res[i * NT + t];
+            }
+#pragma unroll
+            for (int u = 0; u < HOIST && c + u < K; ++u) m = (rr[u] > 0 && hn[u] < m) ? hn[u] : m;
+          }
+          if (m >= hv) {
+            hv = (m >= INF_H - 1) ? INF_H : m + 1;
+            s_h[hi] = hv;
+          }
+        }
+        // list the active nodes (ex > 0, label < INF) with one LDS atomic per wave
+        const bool act = e > 0 && hv < INF_H;
         const unsigned long long b = __ballot(act);
         const int lane = threadIdx.x & 63;
         int base = 0;
@@ -483,59 +572,17 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         if (e != e0) lds_add(&s_ex[t], e - e0);
       }
       __syncthreads();
-
-      // relabel listed nodes that kept excess, residuals frozen; labels only
-      // grow, so concurrent reads of a neighbour's label are safe
-      for (int q = threadIdx.x; q < n; q += NTHREADS) {
-        const int t = s_list[q];
-        if (s_ex[t] <= 0) continue;
-        int z, y, x;
-        tile_coords<G>(t, z, y, x);
-        const int hi = ((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX);
-        const int hv = s_h[hi];
-        int m = INF_H;
-#pragma unroll
-        for (int c = 0; c < K; c += HOIST) {
-          int hn[HOIST];
-          T rr[HOIST];
-#pragma unroll
-          for (int u = 0; u < HOIST && c + u < K; ++u) {
-            const int i = c + u;
-            hn[u] = s_h[hi + (DIR::z(p.dirs, i) * G::HH + DIR::y(p.dirs, i)) * G::HW + DIR::x(p.dirs, i)];
-            rr[u] = s_res[i * NT + t];
-          }
-#pragma unroll
-          for (int u = 0; u < HOIST && c + u < K; ++u) m = (rr[u] > 0 && hn[u] < m) ? hn[u] : m;
-        }
-        if (m >= hv) s_h[hi] = (m >= INF_H - 1) ? INF_H : m + 1;
-      }
-      __syncthreads();
     }
 
-    // write back the tile's residual planes, excess, labels and arc masks
-    // (whole: tracking which words changed measured slower, nearly every node
-    // of a processed tile changes in a round)
+    // the tile stays listed while it holds active nodes (nodes past the
+    // lattice edge hold no excess)
     bool act = false;
 #pragma unroll
     for (int j = 0; j < G::NPT; ++j) {
       const int t = threadIdx.x + j * NTHREADS;
       int z, y, x;
       tile_coords<G>(t, z, y, x);
-      if (z >= ld || y >= lh || x >= lw) continue;
-      const T e = s_ex[t];
-      const int hv = s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)];
-      const long long g = ((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x);
-      MaskT<K> m = 0;
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const T r = s_res[i * NT + t];
-        p.res[i * N + g] = r;
-        if (r > 0) m |= (MaskT<K>)1 << i;
-      }
-      p.ex[g] = e;
-      p.h[g] = hv;
-      reinterpret_cast<MaskT<K>*>(p.mask)[g] = m;
-      if (e > 0 && hv < INF_H) act = true;
+      act |= s_ex[t] > 0 && s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)] < INF_H;
     }
     const unsigned nbr_sides = s_nbr;  // final: the last push phase ended at a barrier
     const bool self_active = __syncthreads_or(act);
@@ -572,6 +619,31 @@ __global__ __launch_bounds__(NTHREADS) void k_push(KP<T> p) {
         base = __shfl(base, first, 64);
         if (claim) p.wl_out[base + __popcll(won & ((1ull << lane) - 1))] = claimed_tile;
       }
+    }
+    // write back the tile's residual planes, excess, labels and arc masks
+    // (whole: tracking which words changed measured slower, nearly every node
+    // of a processed tile changes in a round); the appends above are in
+    // flight meanwhile
+#pragma unroll
+    for (int j = 0; j < G::NPT; ++j) {
+      const int t = threadIdx.x + j * NTHREADS;
+      int z, y, x;
+      tile_coords<G>(t, z, y, x);
+      if (z >= ld || y >= lh || x >= lw) continue;
+      const T e = s_ex[t];
+      const int hv = s_h[((z + G::RZ) * G::HH + (y + G::RY)) * G::HW + (x + G::RX)];
+      const unsigned g = (unsigned)(((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x));
+      const unsigned o = g * (unsigned)sizeof(T);
+      MaskT<K> m = 0;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const T r = s_res[i * NT + t];
+        bst<T>(p.res + (long long)i * N, o, r);
+        if (r > 0) m |= (MaskT<K>)1 << i;
+      }
+      bst<T>(p.ex, o, e);
+      bst<int>(p.h, g * 4u, hv);
+      bst<MaskT<K>>(p.mask, g * (unsigned)sizeof(MaskT<K>), m);
     }
     // no barrier here: the next tile's first barrier follows thread 0's
     // prologue; the appends above use only registers (nbr_sides was read
@@ -1542,6 +1614,7 @@ __global__ __launch_bounds__(NTHREADS) void k_load(KP<T> p, const long long* __r
         z.x = 0;
         z.y = 0;
         *reinterpret_cast<T2*>(p.mbox + (long long)i * p.N + v) = z;
+        *reinterpret_cast<T2*>(p.mbox_in + (long long)i * p.N + v) = z;
       }
     }
     T2 e;
@@ -1562,7 +1635,10 @@ __global__ __launch_bounds__(NTHREADS) void k_load(KP<T> p, const long long* __r
       ma = a > ma ? a : ma;
       if (a > 0) m |= (MaskT<K>)1 << i;
       p.res[(long long)i * p.N + v] = (T)a;
-      if (zero_mbox) p.mbox[(long long)i * p.N + v] = 0;
+      if (zero_mbox) {
+        p.mbox[(long long)i * p.N + v] = 0;
+        p.mbox_in[(long long)i * p.N + v] = 0;
+      }
     }
     p.ex[v] = (T)(-r);
     reinterpret_cast<MaskT<K>*>(p.mask)[v] = m;
@@ -2357,6 +2433,9 @@ template <int K, class G0, class G1, class G2, class B, typename T, typename MT,
 class Engine final : public MinCutEngine {
  public:
   Engine(const LatticeGeom& geo) : g_(geo) {
+    // the push kernel addresses a plane by 32-bit byte offsets
+    if ((unsigned long long)geo.n() * std::max<size_t>(sizeof(T), 4) >= (1ull << 32))
+      fail(LF_ERR_UNSUPPORTED, "lattice too large for one device engine: nodes x element bytes must stay below 4 GiB");
     LF_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     N_ = geo.n();
     p_.D = (int)geo.D; p_.H = (int)geo.H; p_.W = (int)geo.W; p_.N = N_;
@@ -2381,7 +2460,7 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipMalloc(&red0_, sizeof(long long) * N_));
     LF_HIP(hipMalloc(&alpha0_, sizeof(long long) * N_ * K));
     LF_HIP(hipMalloc(&res_, sizeof(T) * N_ * K));
-    LF_HIP(hipMalloc(&mbox_, sizeof(T) * N_ * K));
+    LF_HIP(hipMalloc(&mbox_, sizeof(T) * N_ * K * 2));  // one mailbox per round parity
     LF_HIP(hipMalloc(&ex_, sizeof(T) * N_));
     LF_HIP(hipMalloc(&h_, sizeof(int) * N_));
     LF_HIP(hipMalloc(&mask_, sizeof(MT) * N_));
@@ -2391,8 +2470,8 @@ class Engine final : public MinCutEngine {
     LF_HIP(hipMalloc(&wl_[0], sizeof(int) * ntiles_));
     LF_HIP(hipMalloc(&wl_[1], sizeof(int) * ntiles_));
     LF_HIP(hipMalloc(&wl_cnt_, 4 * sizeof(unsigned)));
-    LF_HIP(hipMalloc(&mail_, sizeof(unsigned) * ntiles_));
-    LF_HIP(hipMemsetAsync(mail_, 0, sizeof(unsigned) * ntiles_, s_));
+    LF_HIP(hipMalloc(&mail_, sizeof(unsigned) * ntiles_ * 2));
+    LF_HIP(hipMemsetAsync(mail_, 0, sizeof(unsigned) * ntiles_ * 2, s_));
     LF_HIP(hipMalloc(&dmail_, sizeof(unsigned) * ntiles_));
     LF_HIP(hipMemsetAsync(dmail_, 0, sizeof(unsigned) * ntiles_, s_));
     LF_HIP(hipMemsetAsync(wl_cnt_, 0, 4 * sizeof(unsigned), s_));
@@ -3169,7 +3248,8 @@ class Engine final : public MinCutEngine {
  private:
   KP<T> params() const {
     KP<T> p = p_;
-    p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_; p.mail = mail_; p.dmail = dmail_; p.mask = mask_;
+    p.res = res_; p.ex = ex_; p.h = h_; p.mbox = mbox_; p.mbox_in = mbox_ + N_ * K; p.mail = mail_;
+    p.mail_in = mail_ + ntiles_; p.dmail = dmail_; p.mask = mask_;
     p.counter = counter_ptr();
     p.tiles = tiles_ptr();
     p.flag_in = flags_[cur_];
@@ -3358,6 +3438,12 @@ class Engine final : public MinCutEngine {
   // list is the other one, and the three list counters rotate (in, out, to clear).
   void launch_push_round(const KP<T>& p) {
     KP<T> q = p;
+    // the round parity is cur_: this round fills mailbox cur_ and drains the
+    // other, which the round before filled
+    q.mbox = mbox_ + (cur_ ? N_ * K : 0);
+    q.mbox_in = mbox_ + (cur_ ? 0 : N_ * K);
+    q.mail = mail_ + (cur_ ? ntiles_ : 0);
+    q.mail_in = mail_ + (cur_ ? 0 : ntiles_);
     q.flag_in = flags_[cur_];
     q.flag_out = flags_[cur_ ^ 1];
     q.wl_in = wl_[cur_];
@@ -3437,12 +3523,18 @@ class Engine final : public MinCutEngine {
     bool deferred = false;
     // the delivery pass also sets the start labels when the masks are current
     const bool fused_init = pushed && !bits_ && mask_valid_;
-    if (pushed)
+    if (pushed) {
+      // only the last round's mailbox holds flow: the tiles it was sent to
+      // drained the earlier ones in the round after (and a relabel found
+      // every mailbox empty)
+      KP<T> q = p;
+      q.mbox = mbox_ + (cur_ ? 0 : N_ * K);
       with_geo([&](auto g) {
         using GG = decltype(g);
-        hipLaunchKernelGGL((k_deliver<K, GG, T, DIR>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, p,
+        hipLaunchKernelGGL((k_deliver<K, GG, T, DIR>), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, q,
                            fused_init ? 1 : 0);
       });
+    }
     if (pushed) mbox_dirty_ = false;  // every slot the rounds filled is delivered and cleared
     if (bits_) {
       const long long nw = (long long)p_.D * p_.H * wp_.ntx;  // one wave per plane word
@@ -3471,7 +3563,7 @@ class Engine final : public MinCutEngine {
     }
     // push rounds start with exactly the tiles holding active nodes, and with
     // empty mailboxes (delivered above: no mail or delivery bits left)
-    fill_many({{mail_, 0u, ntiles_},
+    fill_many({{mail_, 0u, 2 * ntiles_},
                {pushed ? dmail_ : nullptr, 0u, ntiles_},
                {u32(flags_[cur_]), 0u, ntiles_},
                {u32(flags_[cur_ ^ 1]), 0u, ntiles_},
