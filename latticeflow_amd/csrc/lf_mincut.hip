@@ -1129,9 +1129,12 @@ __device__ inline long long widx(const WP& w, int pl, int z, int ty, int tx, int
 // Bit planes of the residual arcs and sinks from the working state, and the
 // BFS start labels (0 on sink-capable nodes, INF_H elsewhere).  One wave per
 // 64-node word of a row; the word is the wave's ballot.  Plane words of absent
-// directions and of rows past the lattice stay zero from allocation.
+// directions and of rows past the lattice stay zero from allocation.  With
+// use_mask the arcs come from the per-node arc masks (one byte per node
+// instead of k residuals).
 template <int K, typename T>
-__global__ __launch_bounds__(NTHREADS) void k_bits_init(KP<T> p, WP w, unsigned long long* __restrict__ planes) {
+__global__ __launch_bounds__(NTHREADS) void k_bits_init(KP<T> p, WP w, unsigned long long* __restrict__ planes,
+                                                        int use_mask) {
   const long long wave = ((long long)blockIdx.x * NTHREADS + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= (long long)p.D * p.H * w.ntx) return;  // uniform over the wave
@@ -1142,9 +1145,11 @@ __global__ __launch_bounds__(NTHREADS) void k_bits_init(KP<T> p, WP w, unsigned 
   const bool in = x < p.W;
   const long long v = r * p.W + x;
   const long long base = widx(w, 0, z, y >> 6, tx, y & 63);
+  const MaskT<K> m = (use_mask && in) ? reinterpret_cast<const MaskT<K>*>(p.mask)[v] : (MaskT<K>)0;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
-    const unsigned long long b = __ballot(in && p.res[(long long)i * p.N + v] > 0);
+    const bool arc = use_mask ? ((m >> i) & 1) != 0 : (in && p.res[(long long)i * p.N + v] > 0);
+    const unsigned long long b = __ballot(arc);
     if (lane == 0) planes[base + (long long)slot_plane(p.dirs.dy[i], p.dirs.dx[i]) * w.ps] = b;
   }
   const T e = in ? p.ex[v] : (T)0;
@@ -1175,11 +1180,35 @@ __device__ inline void flag_around2d(const BP& p, int tile) {
     }
 }
 
+// Whole-wave lane shifts by one (DPP wave_shr:1 / wave_shl:1, VALU moves
+// instead of LDS permutes): up(v) gives lane i the value of lane i - 1 (lane 0
+// gets 0), down(v) the value of lane i + 1 (lane 63 gets 0).
+__device__ inline unsigned wave_up1(unsigned v) { return __builtin_amdgcn_update_dpp(0u, v, 0x138, 0xf, 0xf, false); }
+__device__ inline unsigned wave_down1(unsigned v) { return __builtin_amdgcn_update_dpp(0u, v, 0x130, 0xf, 0xf, false); }
+__device__ inline unsigned long long wave_up1(unsigned long long v) {
+  return ((unsigned long long)wave_up1((unsigned)(v >> 32)) << 32) | wave_up1((unsigned)v);
+}
+__device__ inline unsigned long long wave_down1(unsigned long long v) {
+  return ((unsigned long long)wave_down1((unsigned)(v >> 32)) << 32) | wave_down1((unsigned)v);
+}
+// value of a 64-bit register in lane r (r uniform over the wave)
+__device__ inline unsigned long long lane_u64(unsigned long long v, int r) {
+  return ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(v >> 32), r) << 32) |
+         (unsigned)__builtin_amdgcn_readlane((int)v, r);
+}
+
 // One wave tile of the exact distance labelling (same fixpoint as k_bfs_tile):
 // the tile's labels are recomputed level by level from its sinks and from the
 // current labels of its halo, d(v) = 1 + min d(w) over residual arcs v -> w.
 // Halo labels only decrease from launch to launch, so the recomputed labels
-// never exceed the previous ones; a changed border label flags the neighbours.
+// never exceed the previous ones (a node the recomputation does not reach had
+// no finite label); a changed border label flags the neighbours.
+//
+// A level costs a few word operations per lane; the labels of the nodes it
+// reaches are stored into LDS without reading anything back, either bit by
+// bit in each row or row by row across the lanes, whichever takes fewer
+// stores, and the whole tile is written back at the end with row-coalesced
+// stores.
 __global__ __launch_bounds__(WT) void k_bfs_wave(BP p, WP w, const unsigned long long* __restrict__ planes,
                                                  int* __restrict__ hgl) {
   __shared__ int s_lab[WT * (WT + 1)];
@@ -1198,6 +1227,7 @@ __global__ __launch_bounds__(WT) void k_bfs_wave(BP p, WP w, const unsigned long
   const int y = y0 + lane, xc = x0 + lane;
   const bool rowok = y < H, colok = xc < W;
   const bool up = y0 > 0, dn = y0 + WT < H, lf = x0 > 0, rt = x0 + WT < W;
+  const bool full_r = y0 + WT <= H, full_c = x0 + WT <= W;  // the tile has its last row / column
   unsigned long long P[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) P[s] = rowok ? planes[widx(w, s, tz, ty, tx, lane)] : 0ull;
@@ -1212,11 +1242,12 @@ __global__ __launch_bounds__(WT) void k_bfs_wave(BP p, WP w, const unsigned long
   const int ctr = (up && rt) ? hgl[rtop + x0 + WT] : INF_H;
   const int cbl = (dn && lf) ? hgl[rbot + x0 - 1] : INF_H;
   const int cbr = (dn && rt) ? hgl[rbot + x0 + WT] : INF_H;
-  for (int r = 0; r < WT; ++r) {
-    const int yy = y0 + r;
-    s_lab[r * (WT + 1) + lane] = (yy < H && colok) ? hgl[(lay + yy) * W + xc] : INF_H;
-  }
-  __syncthreads();
+  // the tile's own border labels before this launch (row 0 / 63 by column,
+  // column 0 / 63 by row), to tell whether a border label changed
+  const int o_top = colok ? hgl[(lay + y0) * W + xc] : INF_H;
+  const int o_bot = (colok && full_r) ? hgl[(lay + y0 + WT - 1) * W + xc] : INF_H;
+  const int o_lft = rowok ? hgl[ry + x0] : INF_H;
+  const int o_rgt = (rowok && full_c) ? hgl[ry + x0 + WT - 1] : INF_H;
 
   // lowest finite halo label >= a (uniform over the wave), INF_H if none
   auto next_level = [&](int a) {
@@ -1231,21 +1262,21 @@ __global__ __launch_bounds__(WT) void k_bfs_wave(BP p, WP w, const unsigned long
     c = (cbr >= a && cbr < c) ? cbr : c;
     return wave_min_int(c);
   };
-  unsigned long long vis = sink, F = sink, chg = 0;
+  unsigned long long vis = sink, F = sink;
   int L = __ballot(sink != 0ull) ? 0 : next_level(0);
   while (L < INF_H) {
     // frontier of level L: tile nodes F, halo nodes labelled L
     const unsigned long long topw = __ballot(ht == L), botw = __ballot(hb == L);
-    const int e = (hl == L ? 1 : 0) | (hr == L ? 2 : 0);
-    unsigned long long Fu = __shfl_up(F, 1, 64), Fd = __shfl_down(F, 1, 64);
-    int eu = __shfl_up(e, 1, 64), ed = __shfl_down(e, 1, 64);
+    const unsigned e = (hl == L ? 1u : 0u) | (hr == L ? 2u : 0u);
+    unsigned long long Fu = wave_up1(F), Fd = wave_down1(F);
+    unsigned eu = wave_up1(e), ed = wave_down1(e);
     if (lane == 0) {
       Fu = topw;
-      eu = (ctl == L ? 1 : 0) | (ctr == L ? 2 : 0);
+      eu = (ctl == L ? 1u : 0u) | (ctr == L ? 2u : 0u);
     }
     if (lane == WT - 1) {
       Fd = botw;
-      ed = (cbl == L ? 1 : 0) | (cbr == L ? 2 : 0);
+      ed = (cbl == L ? 1u : 0u) | (cbr == L ? 2u : 0u);
     }
     // node (y, x) joins level L+1 when its arc towards (y+dy, x+dx) is
     // residual and that node is in the frontier
@@ -1261,25 +1292,46 @@ __global__ __launch_bounds__(WT) void k_bfs_wave(BP p, WP w, const unsigned long
     const unsigned long long nw = c & ~vis;
     vis |= nw;
     ++L;
-    for (unsigned long long b = nw; b; b &= b - 1) {
-      const int x = __builtin_ctzll(b);
-      int* q = &s_lab[lane * (WT + 1) + x];
-      if (*q != L) {
-        *q = L;
-        chg |= 1ull << x;
+    const unsigned long long rows = __ballot(nw != 0ull);
+    if (rows) {
+      // few rows reached (a front along a row): the lanes store one row at a
+      // time; otherwise each lane stores its own row's new bits
+      if (__popcll(rows) <= 4) {
+        for (unsigned long long rr = rows; rr; rr &= rr - 1) {
+          const int r = __builtin_ctzll(rr);
+          if ((lane_u64(nw, r) >> lane) & 1ull) s_lab[r * (WT + 1) + lane] = L;
+        }
+      } else {
+        for (unsigned long long b = nw; b; b &= b - 1) s_lab[lane * (WT + 1) + __builtin_ctzll(b)] = L;
       }
     }
     F = nw;
-    if (!__any(nw != 0ull)) L = next_level(L);
+    if (!rows) L = next_level(L);
   }
   __syncthreads();
-  const unsigned long long bord = (lane == 0 || lane == WT - 1) ? chg : (chg & 0x8000000000000001ull);
-  const bool border = __any(bord != 0ull);
+  // write back every label of the tile (row r, column lane): sinks 0, reached
+  // nodes their level, the rest INF_H
+  bool border = false;
   for (int r = 0; r < WT; ++r) {
-    const unsigned long long cr = __shfl(chg, r, 64);
-    if ((cr >> lane) & 1ull) hgl[(lay + y0 + r) * W + xc] = s_lab[r * (WT + 1) + lane];
+    const int yy = y0 + r;
+    if (yy >= H) break;
+    const unsigned long long vr = lane_u64(vis, r), sr = lane_u64(sink, r);
+    const int lab = ((sr >> lane) & 1ull) ? 0 : (((vr >> lane) & 1ull) ? s_lab[r * (WT + 1) + lane] : INF_H);
+    if (colok) {
+      hgl[(lay + yy) * W + xc] = lab;
+      if (r == 0 && lab != o_top) border = true;
+      if (r == WT - 1 && lab != o_bot) border = true;
+    }
   }
-  if (border && lane == 0) flag_around2d(p, tile);
+  if (rowok) {
+    const int l0 = ((sink & 1ull) ? 0 : ((vis & 1ull) ? s_lab[lane * (WT + 1)] : INF_H));
+    if (l0 != o_lft) border = true;
+    if (full_c) {
+      const int l1 = ((sink >> 63) ? 0 : ((vis >> 63) ? s_lab[lane * (WT + 1) + WT - 1] : INF_H));
+      if (l1 != o_rgt) border = true;
+    }
+  }
+  if (__any(border) && lane == 0) flag_around2d(p, tile);
 }
 
 // One wave tile of the S_min flood (same fixpoint as k_flood_tile): the
@@ -1308,6 +1360,7 @@ __global__ __launch_bounds__(WT) void k_flood_wave(BP p, WP w, const unsigned lo
   for (int s = 0; s < 8; ++s) P[s] = rowok ? planes[widx(w, s, tz, ty, tx, lane)] : 0ull;
   // reached nodes of the tile, one ballot per row
   unsigned long long R = 0;
+#pragma unroll 16
   for (int r = 0; r < WT; ++r) {
     const int yy = y0 + r;
     const unsigned long long b = __ballot(yy < H && colok && reach[(lay + yy) * W + xc] != 0);
@@ -1378,9 +1431,7 @@ __global__ __launch_bounds__(WT) void k_flood_wave(BP p, WP w, const unsigned lo
                                      ((R & P[slot_plane(1, 1)]) << 1);
     const unsigned long long to_up = ((R & P[slot_plane(-1, -1)]) >> 1) | (R & P[slot_plane(-1, 0)]) |
                                      ((R & P[slot_plane(-1, 1)]) << 1);
-    unsigned long long from_up = __shfl_up(to_dn, 1, 64), from_dn = __shfl_down(to_up, 1, 64);
-    if (lane == 0) from_up = 0;
-    if (lane == WT - 1) from_dn = 0;
+    const unsigned long long from_up = wave_up1(to_dn), from_dn = wave_down1(to_up);
     const unsigned long long Rn = R | from_up | from_dn;
     if (!__any(Rn != R)) break;
     R = Rn;
@@ -1389,7 +1440,7 @@ __global__ __launch_bounds__(WT) void k_flood_wave(BP p, WP w, const unsigned lo
   const unsigned long long bord = (lane == 0 || lane == WT - 1) ? add : (add & 0x8000000000000001ull);
   const bool border = __any(bord != 0ull);
   for (int r = 0; r < WT; ++r) {
-    const unsigned long long a = __shfl(add, r, 64);
+    const unsigned long long a = lane_u64(add, r);
     if ((a >> lane) & 1ull) reach[(lay + y0 + r) * W + xc] = 1;
   }
   if (border && lane == 0) flag_around2d(p, tile);
@@ -3540,7 +3591,7 @@ class Engine final : public MinCutEngine {
       const long long nw = (long long)p_.D * p_.H * wp_.ntx;  // one wave per plane word
       if (!resume)
         hipLaunchKernelGGL((k_bits_init<K, T>), dim3((unsigned)((nw * 64 + NTHREADS - 1) / NTHREADS)),
-                           dim3(NTHREADS), 0, s_, p, wp_, planes_);
+                           dim3(NTHREADS), 0, s_, p, wp_, planes_, mask_valid_ ? 1 : 0);
       *exact = run_tiled([&](const BP& q) {
         hipLaunchKernelGGL(k_bfs_wave, dim3(bp_.nblocks), dim3(WT), 0, s_, q, wp_, planes_, h_);
         ++stats.bfs_launches;
