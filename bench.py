@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # (rounds per relabel, local iterations, tile shape, relabel launches), fixed
 # so that the line and its rocprofv3 profile run the same schedule;
 # --autotune runs the search instead.
-DEFAULT_TUNING = {"seg8": (4, 4, 1, 2), "blobs26": (4, 4, 1, 2), "grid4": (0, 0, 3, 0), "noise26": (0, 0, 1, 0)}
+DEFAULT_TUNING = {"seg8": (4, 8, 1, 2), "blobs26": (4, 2, 1, 2), "grid4": (6, 4, 3, 2), "noise26": (0, 0, 1, 0)}
 
 
 def make_workload(name, shape):
@@ -148,6 +148,27 @@ def main():
     bfs_achieved = bfs_bytes / bfs_s / 1e9 if bfs_s > 0 else 0.0
     traffic, traffic_src = measured_traffic("%s %s %s" % (a.workload, "x".join(map(str, shape)), w["kernel"]),
                                             tuning, with_source=True)
+    # the same kernel under the schedule with the least LDS work per staged
+    # byte (4 local iterations per round): the fraction of the HBM roofline
+    # a round reaches when its time goes mostly to staging the tile
+    push4 = None
+    if a.workload == "seg8" and not explicit and not a.autotune:
+        lf.setTuning(rounds_per_relabel=4, local_iters=4, tile_shape=tuning["tile_shape"],
+                     relabel_launches=tuning["relabel_launches"])
+        lf.run()
+        p_s, p_t, p_l = 0.0, 0, 0
+        for _ in range(3):
+            lf.run()
+            s4 = lf.stats()
+            p_s += s4["push_seconds"]
+            p_t += s4["tiles_pushed"]
+            p_l += s4["rounds"]
+        lf.setTuning(**{k: tuning[k] for k in ("rounds_per_relabel", "local_iters", "tile_shape", "relabel_launches")})
+        b4 = p_t * s4["tile_nodes"] * s4["bytes_per_node"]
+        push4 = {"local_iters": 4, "rounds_per_relabel": 4, "avg_launch_us": round(p_s / max(p_l, 1) * 1e6, 2),
+                 "nodes_per_launch": round(p_t * s4["tile_nodes"] / max(p_l, 1), 1),
+                 "achieved": round(b4 / p_s / 1e9, 1) if p_s > 0 else 0.0,
+                 "frac": round(b4 / p_s / 1e9 / HBM_PEAK_GBS, 4) if p_s > 0 else 0.0}
     roofline = {
         "bound": "hbm",
         "kernel": "k_push (push/relabel round)",
@@ -172,6 +193,8 @@ def main():
         "step": {"achieved": round((bytes_alg + bfs_bytes) / a.steps / dt / 1e9, 1),
                  "frac": round((bytes_alg + bfs_bytes) / a.steps / dt / 1e9 / HBM_PEAK_GBS, 4)},
     }
+    if push4:
+        roofline["k_push_at_4_local_iters"] = push4
 
     cpu = None
     parity = None
