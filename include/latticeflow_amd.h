@@ -140,6 +140,27 @@ int lf_get_tuning(const lf_handle *h, lf_tuning *tuning);
  * candidate yields the same flow and cuts. */
 int lf_autotune(lf_handle *h, int reps, lf_tuning *best);
 int lf_run(lf_handle *h);
+/* Invariant checks of every solve of this handle (on: nonzero): after each
+ * chunk of push rounds and each exact global relabel the working state is
+ * copied to the host and checked -- residuals >= 0, each arc pair's residuals
+ * plus the flow in flight equal its capacity, flow conservation at every
+ * node, arc masks, and exact distance labels after an exact relabel; the first
+ * violation fails the solve with LF_ERR_STATE and lf_last_error names it.
+ * Replaces the reference's compile-time debug verification
+ * (solver/network_flow/push_relabel.hpp:132-267, _debug_forceVerifyAll);
+ * for small lattices (every check copies the whole state). */
+int lf_set_debug_checks(lf_handle *h, int on);
+/* The same checks on host arrays in the device engine's layout (no device
+ * needed): lattice (D, H, W) = dhw[0..2], k directions dirs[3*i + {0,1,2}] =
+ * (dz, dy, dx) with entry i + k/2 the reverse of entry i; red0[N] and
+ * alpha0[k*N] (plane-major) the base network; res[k*N], ex[N] of cap_bytes
+ * each; mbox: mbox_planes planes of k*N entries of flow in flight (may be
+ * NULL with 0 planes); mask[N] of mask_bytes (NULL: not checked); h[N]
+ * labels, checked as exact distances when labels_exact.  LF_OK, or
+ * LF_ERR_STATE with the first violation in lf_last_error. */
+int lf_verify_state(const int64_t *dhw, int k, const int32_t *dirs, const int64_t *red0, const int64_t *alpha0,
+                    int cap_bytes, const void *res, const void *ex, const void *mbox, int mbox_planes,
+                    const void *mask, int mask_bytes, const int32_t *h, int labels_exact);
 /* runTime() (interface.hpp:115): wall seconds of the last lf_run. */
 int lf_run_time(const lf_handle *h, double *seconds);
 int lf_get_stats(const lf_handle *h, lf_stats *stats);

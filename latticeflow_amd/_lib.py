@@ -108,6 +108,8 @@ _SIG = {
     "lf_get_tuning": (_I, [_P, ctypes.POINTER(lf_tuning)]),
     "lf_autotune": (_I, [_P, _I, ctypes.POINTER(lf_tuning)]),
     "lf_run": (_I, [_P]),
+    "lf_set_debug_checks": (_I, [_P, _I]),
+    "lf_verify_state": (_I, [_P, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _I, _P, _I]),
     "lf_run_time": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "lf_get_stats": (_I, [_P, ctypes.POINTER(lf_stats)]),
     "lf_get_cut": (_I, [_P, _P, _I]),

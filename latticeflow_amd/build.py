@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblatticeflow_amd.so")
 SOURCES = ["lf_mincut.hip", "lf_mincut_canon.hip", "lf_mincut_generic.hip", "lf_mincut_wide.hip", "lf_mincut_wide32.hip", "lf_mincut_wide48.hip", "lf_tvnet.hip",
-           "lf_tv.cpp", "lf_tvpath_core.cpp", "lf_tvpath.cpp", "lf_bk.cpp", "lf_capi.cpp"]
+           "lf_tv.cpp", "lf_tvpath_core.cpp", "lf_tvpath.cpp", "lf_bk.cpp", "lf_verify.cpp", "lf_capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -23,6 +23,7 @@
 #include "lf_kernels.hpp"
 #include "lf_tv.hpp"
 #include "lf_tvpath.hpp"
+#include "lf_verify.hpp"
 
 using namespace lfamd;
 
@@ -285,6 +286,40 @@ int lf_set_tuning(lf_handle* h, const lf_tuning* t) {
   return guarded([&] {
     if (!h) fail(LF_ERR_BAD_ARG, "null argument");
     h->tuning = t ? *t : lf_tuning{};
+  });
+}
+
+int lf_set_debug_checks(lf_handle* h, int on) {
+  return guarded([&] {
+    if (!h) fail(LF_ERR_BAD_ARG, "null argument");
+    if (h->eng) h->eng->set_debug_checks(on != 0);
+  });
+}
+
+int lf_verify_state(const int64_t* dhw, int k, const int32_t* dirs, const int64_t* red0, const int64_t* alpha0,
+                    int cap_bytes, const void* res, const void* ex, const void* mbox, int mbox_planes,
+                    const void* mask, int mask_bytes, const int32_t* h, int labels_exact) {
+  return guarded([&] {
+    if (!dhw || !dirs || !red0 || !alpha0 || !res || !ex || k <= 0 || k % 2 || k > 64)
+      fail(LF_ERR_BAD_ARG, "bad state arrays");
+    if (cap_bytes != 4 && cap_bytes != 8) fail(LF_ERR_BAD_ARG, "cap_bytes must be 4 or 8");
+    if (mask && mask_bytes != 1 && mask_bytes != 2 && mask_bytes != 4 && mask_bytes != 8)
+      fail(LF_ERR_BAD_ARG, "mask_bytes must be 1, 2, 4 or 8");
+    if ((mbox_planes != 0 && !mbox) || mbox_planes < 0) fail(LF_ERR_BAD_ARG, "bad mailbox planes");
+    std::vector<int> dz(k), dy(k), dx(k);
+    for (int i = 0; i < k; ++i) {
+      dz[i] = dirs[3 * i];
+      dy[i] = dirs[3 * i + 1];
+      dx[i] = dirs[3 * i + 2];
+    }
+    VerifyView v;
+    v.D = (int)dhw[0]; v.H = (int)dhw[1]; v.W = (int)dhw[2]; v.K = k;
+    v.dz = dz.data(); v.dy = dy.data(); v.dx = dx.data();
+    v.red0 = red0; v.alpha0 = alpha0; v.cap_bytes = cap_bytes;
+    v.res = res; v.ex = ex; v.mbox = mbox; v.mbox_planes = mbox_planes;
+    v.mask = mask; v.mask_bytes = mask_bytes; v.h = h; v.labels_exact = labels_exact != 0;
+    const std::string err = verify_state(v);
+    if (!err.empty()) fail(LF_ERR_STATE, err);
   });
 }
 

@@ -94,6 +94,10 @@ class MinCutEngine {
   virtual void batch_flows(int64_t* host_flows) = 0;  // raw flow of every lattice of the last run
   // Cuts of every lattice, concatenated in batch node order.
   virtual void batch_cut_device(int32_t* d_out, int which, hipStream_t s) = 0;
+  // Invariant checks of the working state on host copies after every push
+  // chunk and exact relabel (lf_verify.hpp); a violation fails the solve with
+  // LF_ERR_STATE.  Engines without the checks ignore the switch.
+  virtual void set_debug_checks(bool on) { (void)on; }
   int64_t flow = 0;
   bool solved = false;
   bool pf_solved = false;

@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "lf_engine.hpp"
+#include "lf_verify.hpp"
 
 namespace lfamd {
 namespace {
@@ -2581,6 +2582,52 @@ class Engine final : public MinCutEngine {
 
   hipStream_t stream() const override { return s_; }
 
+  void set_debug_checks(bool on) override { debug_checks_ = on; }
+
+  // Host copies of the working state checked against the base network
+  // (lf_verify.hpp).  Parametric solves change excesses and cut arcs between
+  // regions, so there only residual signs, masks and labels are checked.
+  void debug_check(bool labels_exact, const char* where) {
+    if (!debug_checks_) return;
+    LF_HIP(hipStreamSynchronize(s_));
+    const size_t nk = (size_t)N_ * K;
+    std::vector<T> res(nk), ex(N_), mb(2 * nk);
+    std::vector<int32_t> h(N_);
+    std::vector<MT> mk(N_);
+    std::vector<int64_t> a0(nk), r0(N_);
+    LF_HIP(hipMemcpy(res.data(), res_, sizeof(T) * nk, hipMemcpyDeviceToHost));
+    LF_HIP(hipMemcpy(ex.data(), ex_, sizeof(T) * N_, hipMemcpyDeviceToHost));
+    LF_HIP(hipMemcpy(mb.data(), mbox_, sizeof(T) * 2 * nk, hipMemcpyDeviceToHost));
+    LF_HIP(hipMemcpy(h.data(), h_, sizeof(int32_t) * N_, hipMemcpyDeviceToHost));
+    LF_HIP(hipMemcpy(mk.data(), mask_, sizeof(MT) * N_, hipMemcpyDeviceToHost));
+    LF_HIP(hipMemcpy(a0.data(), alpha0_, sizeof(int64_t) * nk, hipMemcpyDeviceToHost));
+    LF_HIP(hipMemcpy(r0.data(), red0_, sizeof(int64_t) * N_, hipMemcpyDeviceToHost));
+    std::string err;
+    if (pf_mode_) {
+      for (size_t j = 0; j < nk && err.empty(); ++j)
+        if (res[j] < 0) err = "negative residual at plane entry " + std::to_string(j);
+      for (int64_t v = 0; v < N_ && err.empty() && mask_valid_; ++v)
+        for (int i = 0; i < K; ++i)
+          if ((((unsigned long long)mk[v] >> i) & 1ull) != (res[(size_t)i * N_ + v] > 0 ? 1ull : 0ull)) {
+            err = "arc mask at node " + std::to_string(v) + " direction " + std::to_string(i);
+            break;
+          }
+    } else {
+      VerifyView vv;
+      vv.D = p_.D; vv.H = p_.H; vv.W = p_.W; vv.K = K;
+      vv.dz = g_.dz.data(); vv.dy = g_.dy.data(); vv.dx = g_.dx.data();
+      vv.red0 = r0.data(); vv.alpha0 = a0.data();
+      vv.cap_bytes = (int)sizeof(T);
+      vv.res = res.data(); vv.ex = ex.data(); vv.mbox = mb.data(); vv.mbox_planes = 2;
+      vv.mask = mask_valid_ ? mk.data() : nullptr;
+      vv.mask_bytes = (int)sizeof(MT);
+      vv.h = labels_exact ? h.data() : nullptr;
+      vv.labels_exact = labels_exact;
+      err = verify_state(vv);
+    }
+    if (!err.empty()) fail(LF_ERR_STATE, std::string("state check after ") + where + ": " + err);
+  }
+
   // Push tiling of a solve: the tuning's tile shape, else the shorter tiles
   // (G1) for 2D int64 networks, whose residual planes take twice the LDS (the
   // 32-lambda 1024^2 TV path measured 13.4 s there against 21.5 s on G0),
@@ -2700,6 +2747,7 @@ class Engine final : public MinCutEngine {
       const long long bl0 = stats.bfs_launches;
       bool exact = false;
       const long long active = global_relabel(p, pushed, need_exact ? 0 : relabel_cap, &exact, resume);
+      debug_check(exact, "a global relabel");
       resume = false;
       pushed = false;  // the mailbox was delivered
       double t_relabel = std::chrono::duration<double>(clk::now() - tb).count();
@@ -2748,6 +2796,7 @@ class Engine final : public MinCutEngine {
         LF_HIP(hipGetLastError());
         LF_HIP(hipEventRecord(ev_[1], s_));
         pushed = true;
+        debug_check(false, "push rounds");
         if (fixed_r && done_rounds + n >= R) {
           // the cycle's last chunk of a fixed schedule: its time is read after
           // the next relabel's synchronisation, so the host does not wait here
@@ -3668,6 +3717,7 @@ class Engine final : public MinCutEngine {
   unsigned* mail_ = nullptr;         // per tile: sides that sent it flow
   unsigned* dmail_ = nullptr;        // per tile: sides that sent it flow since the last delivery
   bool mbox_dirty_ = true;           // the mailbox may hold flow (or was never cleared)
+  bool debug_checks_ = false;        // invariant checks after push chunks and relabels
   bool mask_valid_ = false;          // mask_ matches res_ (kept by load, push, delivery, cuts)
   const bool use_graphs_ = std::getenv("LF_NO_GRAPHS") == nullptr;
   std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs_;  // push chunks by (tiling, iters, phase, n)

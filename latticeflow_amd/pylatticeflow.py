@@ -225,6 +225,12 @@ class LatticeFlow(object):
         check(lib().lf_get_tuning(self._h, ctypes.byref(t)))
         return t.as_dict()
 
+    def setDebugChecks(self, on=True):
+        """Check the working state's invariants after every push chunk and exact
+        relabel of each solve (lf_set_debug_checks); a violation raises
+        LatticeFlowError.  For small lattices: each check copies the state."""
+        check(lib().lf_set_debug_checks(self._h, 1 if on else 0))
+
     def autotune(self, reps=1):
         """Time every tile shape / relabel schedule on the current energies and keep the fastest."""
         t = _lib.lf_tuning()
@@ -373,6 +379,12 @@ class LatticeFlowBatch(object):
         t = _lib.lf_tuning()
         check(lib().lf_get_tuning(self._h, ctypes.byref(t)))
         return t.as_dict()
+
+    def setDebugChecks(self, on=True):
+        """Check the working state's invariants after every push chunk and exact
+        relabel of each solve (lf_set_debug_checks); a violation raises
+        LatticeFlowError.  For small lattices: each check copies the state."""
+        check(lib().lf_set_debug_checks(self._h, 1 if on else 0))
 
     def autotune(self, reps=1):
         t = _lib.lf_tuning()
@@ -617,3 +629,39 @@ def calculate2dTVPathEvents(X, lambdas, kernel="Star2d_16", return_stats=False):
     path = TVRegPath(X, max(float(np.maximum(lam, 0).max()), 0.0), kernel)
     R = path(np.maximum(lam, 0))
     return (R, path.stats()) if return_stats else R
+
+
+def verify_state(dhw, dirs, red0, alpha0, res, ex, mbox=None, mask=None, h=None, labels_exact=False):
+    """The library's state checks (lf_verify_state) on host arrays in the device
+    engine's layout: dhw = (D, H, W); dirs (k, 3) int offsets in lattice order;
+    red0 (N,) and alpha0 (k, N) int64 base network; res (k, N) and ex (N,) of
+    int32 or int64; mbox (P, k, N) flow in flight or None; mask (N,) unsigned
+    arc bits or None; h (N,) int32 labels or None.  Returns None when every
+    invariant holds, else the library's description of the first violation."""
+    import numpy as np
+    dhw = np.ascontiguousarray(dhw, np.int64)
+    dirs = np.ascontiguousarray(dirs, np.int32).reshape(-1, 3)
+    k = dirs.shape[0]
+    red0 = np.ascontiguousarray(red0, np.int64)
+    alpha0 = np.ascontiguousarray(alpha0, np.int64)
+    res = np.ascontiguousarray(res)
+    cap_bytes = res.dtype.itemsize
+    ex = np.ascontiguousarray(ex, res.dtype)
+    planes = 0
+    if mbox is not None:
+        mbox = np.ascontiguousarray(mbox, res.dtype)
+        planes = mbox.shape[0]
+    if mask is not None:
+        mask = np.ascontiguousarray(mask)
+    if h is not None:
+        h = np.ascontiguousarray(h, np.int32)
+    ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+    rc = lib().lf_verify_state(ptr(dhw), k, ptr(dirs), ptr(red0), ptr(alpha0), cap_bytes, ptr(res), ptr(ex),
+                               ptr(mbox), planes, ptr(mask), 0 if mask is None else mask.dtype.itemsize, ptr(h),
+                               1 if labels_exact else 0)
+    if rc == _lib.LF_OK:
+        return None
+    if rc == _lib.LF_ERR_STATE:
+        return lib().lf_last_error().decode()
+    check(rc)
+

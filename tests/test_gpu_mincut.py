@@ -258,3 +258,28 @@ def test_4d_lattices(oracle, kernel, cap_bytes):
         un, b = _random_energies(rng, shape, offs, cap=40, wcap=15)
         print("case", kernel, cap_bytes, shape, flush=True)
         _check(oracle, kernel, shape, un, b, cap_bytes)
+
+
+@pytest.mark.parametrize("kernel,nd", [("Full2d_8", 2), ("Full2d_4", 2), ("Star3d_26", 3), ("Full3d_6", 3),
+                                       ("Star2d_16", 2)])
+@pytest.mark.parametrize("cap_bytes", [4, 8])
+def test_debug_checks_hold(oracle, kernel, nd, cap_bytes):
+    """Every push chunk and exact relabel of the solve passes the state checks
+    (capacities, arc pairs with the flow in flight, conservation, masks, exact
+    labels), and the results are the oracle's."""
+    from latticeflow_amd import LatticeFlow
+    rng = np.random.default_rng(hash((kernel, cap_bytes, "dbg")) & 0xffff)
+    for trial in range(3):
+        shape = tuple(int(x) for x in rng.integers(8, 70 if nd == 2 else 14, nd))
+        offs, _ = oracle.edge_order(kernel, shape)
+        un, b = _random_energies(rng, shape, offs, cap=60, wcap=20)
+        lf = LatticeFlow(kernel, "LatticeEnergyMinimizer", shape, numerical_precision=1.0, cap_bytes=cap_bytes)
+        lf.setDebugChecks(True)
+        lf.setTuning(rounds_per_relabel=2 + trial, local_iters=1 + 2 * trial, tile_shape=1 + trial % 3)
+        lf.addEnergyInt(un, b)
+        print("case", kernel, cap_bytes, shape, flush=True)
+        lf.run()
+        red, alpha = oracle.reparam(shape, offs, un, b)
+        ref = oracle.prflow(shape, offs, red, alpha)
+        assert lf.flow() == ref["flow"]
+        assert np.array_equal(lf.getCut("smin").reshape(-1), ref["smin"].astype(np.int32))
