@@ -27,6 +27,8 @@
  *   orc_pf_solve       ParametricFlowSolver::run (divide and conquer)
  *                      solver/parametricflow/pf_solver.hpp:33-208 with
  *                      setToMeanReductionPoint pf_flow_node.hpp:331-396
+ *   orc_pf_solve_mode  the same, optionally splitting each section by its
+ *                      S_min instead of the flipped set
  *   orc_tv_single      TVSolver::setup + runSingleLambda
  *                      solver/tv/tv_solver.hpp:71-112,151-176
  *   orc_tv_path        calculate2dTV(..., vector<double>) output,
@@ -40,6 +42,8 @@
  *   orc_cuts          equals the brute-force minimum
  *   orc_pf_solve      Condat's exact 1-D TV (one quantum), the FP64 ROF dual on
  *                     2-D lattices, exact conservation of sum r
+ *   orc_pf_solve_mode the S_min split gives the flipped-set split's levels on
+ *                     seeded 1-D / 2-D / 3-D cases, final ranges span <= 1
  *   orc_tv_single     the FP64 ROF dual through the lambda convention
  *   orc_tv_path       the FP64 ROF dual through the path conventions
  *                     (R - M = 2 s S (u - mean u), s = lambda / lambda_max) and
@@ -764,8 +768,43 @@ static void pr_flip_back_raw(pr_t *p, int64_t n) {
   p->key_state[n] &= ~1u;
 }
 
-static int64_t pf_run(pr_t *p, int64_t *off, int64_t *pts, int64_t npts, int64_t *bound_out, int64_t *nbound) {
+/* S_min of the section just solved: the nodes reachable from nodes with
+ * excess over residual arcs inside the section.  A flipped node w has the
+ * arcs into it negated (pr_flip_raw), so the residual of u -> w is
+ * alpha(u -> w) with its sign turned when w is flipped.  Marks in_s. */
+static void pf_mark_smin(const pr_t *p, const int64_t *pts, int64_t s, int64_t e, uint8_t *in_s, int64_t *queue) {
+  const uint32_t key = p->key;
+  int64_t qh = 0, qt = 0;
+  for (int64_t i = s; i < e; ++i) {
+    int64_t n = pts[i];
+    in_s[n] = 0;
+  }
+  for (int64_t i = s; i < e; ++i) {
+    int64_t n = pts[i];
+    if (pr_excess(p, n) > 0) { in_s[n] = 1; queue[qt++] = n; }
+  }
+  while (qh < qt) {
+    int64_t u = queue[qh++];
+    for (int i = 0; i < p->k; ++i) {
+      int64_t w = u + p->jumps[i];
+      if ((p->key_state[w] & ~1u) != key || in_s[w]) continue;
+      int64_t r = p->alpha[u * p->k + i];
+      if (p->key_state[w] & 1u) r = -r;
+      if (r <= 0) continue;
+      in_s[w] = 1;
+      queue[qt++] = w;
+    }
+  }
+}
+
+/* smin_split: the sides of each section's cut are S_min and its complement
+ * (the canonical minimal cut, as the device splits) instead of the flipped set
+ * the run leaves (pf_solver.hpp:101-152). */
+static int64_t pf_run(pr_t *p, int64_t *off, int64_t *pts, int64_t npts, int64_t *bound_out, int64_t *nbound,
+                      int smin_split) {
   range_t *q = (range_t *)malloc(sizeof(range_t) * (size_t)(2 * npts + 4));
+  uint8_t *in_s = smin_split ? (uint8_t *)calloc((size_t)p->np, 1) : NULL;
+  int64_t *bq = smin_split ? (int64_t *)malloc(sizeof(int64_t) * (size_t)(npts + 1)) : NULL;
   int64_t qh = 0, qt = 0, nb = 0;
   unsigned key = 1;
   q[qt++] = (range_t){0, npts};
@@ -782,9 +821,10 @@ static int64_t pf_run(pr_t *p, int64_t *off, int64_t *pts, int64_t npts, int64_t
     pr_hot_start(p, pts + rg.start, rg.end - rg.start);
     pr_quick_flow(p);
     pr_run_loop(p);
+    if (smin_split) pf_mark_smin(p, pts, rg.start, rg.end, in_s, bq);
     /* in-place separation by state (pf_solver.hpp:101-152) */
     int64_t fwd = rg.start, bck = rg.end - 1, mid = 0;
-#define ST(i_) ((int)(p->key_state[pts[(i_)]] & 1u))
+#define ST(i_) (smin_split ? (int)in_s[pts[(i_)]] : (int)(p->key_state[pts[(i_)]] & 1u))
     for (;;) {
       int done = 0;
       while (ST(fwd) == 0) {
@@ -813,7 +853,9 @@ static int64_t pf_run(pr_t *p, int64_t *off, int64_t *pts, int64_t npts, int64_t
     if (mid > 0) q[qt++] = (range_t){rg.start, rg.start + mid};
     if (mid < top) {
       q[qt++] = (range_t){rg.start + mid, rg.end};
-      for (int64_t i = rg.start + mid; i < rg.end; ++i) pr_flip_back_raw(p, pts[i]);
+      /* (S_min split: the cleanup below flips back exactly the flipped nodes) */
+      if (!smin_split)
+        for (int64_t i = rg.start + mid; i < rg.end; ++i) pr_flip_back_raw(p, pts[i]);
     }
     if (mid > 0 && mid < top) bound_out[nb++] = mid + rg.start;
     /* cleanupSection (:1468-1487) */
@@ -824,7 +866,7 @@ static int64_t pf_run(pr_t *p, int64_t *off, int64_t *pts, int64_t npts, int64_t
     }
     ++key;
     if (qt + 2 > 2 * npts + 4) { /* cannot happen: every queued range is nonempty */
-      free(q);
+      free(q); free(in_s); free(bq);
       return -1;
     }
   }
@@ -835,16 +877,16 @@ static int64_t pf_run(pr_t *p, int64_t *off, int64_t *pts, int64_t npts, int64_t
     off[n] = 0;
   }
   *nbound = nb;
-  free(q);
+  free(q); free(in_s); free(bq);
   return sections;
 }
 
 /* LatticeLevelReductions on the network (red, alpha): final r per node, the
  * level-set index of each node (sorted boundaries), and the number of
  * section solves. */
-int orc_pf_solve(int nd, const int64_t *shape, int k, const int32_t *offs,
-                 const int64_t *red_in, const int64_t *alpha_in,
-                 int64_t *r_out, int64_t *level_out, int64_t *stats) {
+int orc_pf_solve_mode(int nd, const int64_t *shape, int k, const int32_t *offs,
+                      const int64_t *red_in, const int64_t *alpha_in,
+                      int64_t *r_out, int64_t *level_out, int64_t *stats, int smin_split) {
   ulat_t L = {0};
   if (ulat_init(&L, nd, shape, k, offs)) return -1;
   pr_t p;
@@ -861,7 +903,7 @@ int orc_pf_solve(int nd, const int64_t *shape, int k, const int32_t *offs,
     for (int i = 0; i < k; ++i) p.alpha[w * k + i] = alpha_in[v * k + i];
   }
   int64_t nb = 0;
-  int64_t sections = L.n > 0 ? pf_run(&p, off, pts, L.n, bnd, &nb) : 0;
+  int64_t sections = L.n > 0 ? pf_run(&p, off, pts, L.n, bnd, &nb, smin_split) : 0;
   if (sections < 0) return -4;
   /* level index per node: sort boundaries, map ranges */
   for (int64_t a = 1; a < nb; ++a) {
@@ -880,6 +922,12 @@ int orc_pf_solve(int nd, const int64_t *shape, int k, const int32_t *offs,
   if (stats) { stats[0] = sections; stats[1] = nb; }
   free(lev_pad); free(map); free(pts); free(off); free(bnd); pr_free(&p);
   return 0;
+}
+
+int orc_pf_solve(int nd, const int64_t *shape, int k, const int32_t *offs,
+                 const int64_t *red_in, const int64_t *alpha_in,
+                 int64_t *r_out, int64_t *level_out, int64_t *stats) {
+  return orc_pf_solve_mode(nd, shape, k, offs, red_in, alpha_in, r_out, level_out, stats, 0);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -162,12 +162,13 @@ def lib():
         L.orc_dinic.argtypes = [ci, i64p, ci, i32p, i64p, i64p, i64p, i64p, i64p]
         L.orc_cuts.argtypes = [ci, i64p, ci, i32p, i64p, i64p, i64p, i64p, u8p, u8p]
         L.orc_pf_solve.argtypes = [ci, i64p, ci, i32p, i64p, i64p, i64p, i64p, i64p]
+        L.orc_pf_solve_mode.argtypes = [ci, i64p, ci, i32p, i64p, i64p, i64p, i64p, i64p, ci]
         f64p = ctypes.POINTER(ctypes.c_double)
         L.orc_tv_single.argtypes = [ctypes.c_int64, ctypes.c_int64, ci, i32p, f64p, f64p,
                                     ctypes.c_double, f64p, i64p]
         L.orc_tv_path.argtypes = [ctypes.c_int64, ctypes.c_int64, ci, i32p, f64p, f64p, ctypes.c_int64,
                                   f64p, f64p]
-        for f in (L.orc_reparam, L.orc_prflow_solve, L.orc_dinic, L.orc_cuts, L.orc_pf_solve,
+        for f in (L.orc_reparam, L.orc_prflow_solve, L.orc_dinic, L.orc_cuts, L.orc_pf_solve, L.orc_pf_solve_mode,
                   L.orc_tv_single, L.orc_tv_path):
             f.restype = ci
         _LIB = L
@@ -257,8 +258,10 @@ def dinic(shape, offs, red, alpha):
     return dict(flow=int(flow[0]), smin=smin, smax=smax, red=red_o, res=res_o)
 
 
-def pf_solve(shape, offs, red, alpha):
-    """ParametricFlowSolver restatement. Returns dict(r, level, sections, n_levels)."""
+def pf_solve(shape, offs, red, alpha, smin_split=False):
+    """ParametricFlowSolver restatement. Returns dict(r, level, sections, n_levels).
+    smin_split: each section splits by its S_min (the device's rule) instead of
+    by the flipped set the run leaves."""
     shape, offs, k, nd = _lat(shape, offs)
     n = int(np.prod(shape))
     red = np.ascontiguousarray(red, np.int64)
@@ -266,9 +269,10 @@ def pf_solve(shape, offs, red, alpha):
     r = np.zeros(n, np.int64)
     lev = np.zeros(n, np.int64)
     st = np.zeros(2, np.int64)
-    rc = lib().orc_pf_solve(nd, _p(shape, ctypes.c_int64), k, _p(offs, ctypes.c_int32),
-                            _p(red, ctypes.c_int64), _p(alpha, ctypes.c_int64),
-                            _p(r, ctypes.c_int64), _p(lev, ctypes.c_int64), _p(st, ctypes.c_int64))
+    rc = lib().orc_pf_solve_mode(nd, _p(shape, ctypes.c_int64), k, _p(offs, ctypes.c_int32),
+                                 _p(red, ctypes.c_int64), _p(alpha, ctypes.c_int64),
+                                 _p(r, ctypes.c_int64), _p(lev, ctypes.c_int64), _p(st, ctypes.c_int64),
+                                 1 if smin_split else 0)
     assert rc == 0, rc
     return dict(r=r, level=lev, sections=int(st[0]), n_levels=int(st[1]))
 

@@ -47,6 +47,40 @@ def test_level_reductions_vs_oracle(oracle, kernel, nd):
         assert np.array_equal(cut, (r < 0).astype(np.int32))
 
 
+@pytest.mark.parametrize("kernel,nd", [("Full1d_2", 1), ("Full2d_4", 2), ("Full2d_8", 2), ("Star3d_26", 3)])
+def test_level_regions_vs_oracle_smin(oracle, kernel, nd):
+    """Against the oracle splitting every section by S_min as the device does:
+    on each final range of the oracle the device values have the same exact
+    sum and, sorted, are the same values (a final range holds two adjacent
+    values, so its count and sum fix the multiset; only which node holds which
+    value depends on the max flow each solver found)."""
+    rng = np.random.default_rng(21 + nd)
+    for trial in range(4):
+        if nd == 1:
+            shape = (int(rng.integers(2, 200)),)
+        elif nd == 2:
+            shape = tuple(int(x) for x in rng.integers(2, 40, 2))
+        else:
+            shape = tuple(int(x) for x in rng.integers(2, 10, 3))
+        offs, _ = oracle.edge_order(kernel, shape)
+        n = int(np.prod(shape))
+        kp = len(offs) // 2
+        un = rng.integers(0, 10 ** 6, (n, 2))
+        b = np.zeros((n, kp, 4), np.int64)
+        w = rng.integers(0, 2 * 10 ** 5, (n, kp))
+        b[:, :, 1] = w
+        b[:, :, 2] = w
+        print("case", kernel, shape, flush=True)
+        r, cut, st = _levels_gpu(kernel, shape, (un, b))
+        red, alpha = oracle.reparam(shape, offs, un, b)
+        ref = oracle.pf_solve(shape, offs, red, alpha, smin_split=True)
+        for lv in np.unique(ref["level"]):
+            sel = ref["level"] == lv
+            assert r[sel].sum() == ref["r"][sel].sum(), (kernel, shape, lv)
+            assert np.array_equal(np.sort(r[sel]), np.sort(ref["r"][sel])), (kernel, shape, lv)
+        assert np.abs(r - ref["r"]).max() <= 1
+
+
 @pytest.mark.parametrize("kernel", ["Full2d_8", "Star2d_16", "Full2d_4"])
 def test_tv_single_vs_oracle(oracle, kernel):
     from latticeflow_amd.pylatticeflow import _tv_single

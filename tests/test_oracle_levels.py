@@ -185,3 +185,31 @@ def test_planted_mistakes_caught(oracle):
     # a wrong sign of the unary term
     bad = oracle.pf_solve((n,), offs, -r, alpha)["r"]
     assert np.abs(bad - x).max() > 1.0
+
+
+def test_pf_smin_split_mode(oracle):
+    """The oracle's divide and conquer with sections split by S_min (the
+    device's rule) gives the same levels as the reference's split by the
+    flipped set on these seeded cases, the same exact sums, and final ranges
+    whose values differ by at most one (the stopping rule)."""
+    rng = np.random.default_rng(11)
+    for kernel, shape in [("Full1d_2", (140,)), ("Full2d_4", (21, 26)), ("Full2d_8", (25, 19)),
+                          ("Star3d_26", (6, 5, 7))]:
+        for _ in range(3):
+            offs, _ = oracle.edge_order(kernel, shape)
+            n = int(np.prod(shape))
+            kp = len(offs) // 2
+            un = rng.integers(0, 10 ** 6, (n, 2))
+            b = np.zeros((n, kp, 4), np.int64)
+            w = rng.integers(0, 2 * 10 ** 5, (n, kp))
+            b[:, :, 1] = w
+            b[:, :, 2] = w
+            red, alpha = oracle.reparam(shape, offs, un, b)
+            f = oracle.pf_solve(shape, offs, red, alpha)
+            s = oracle.pf_solve(shape, offs, red, alpha, smin_split=True)
+            assert s["r"].sum() == red.sum()
+            assert np.array_equal(f["r"], s["r"]), (kernel, shape)
+            for lv in np.unique(s["level"]):
+                rr = s["r"][s["level"] == lv]
+                assert rr.max() - rr.min() <= 1
+
