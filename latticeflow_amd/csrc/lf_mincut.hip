@@ -2554,7 +2554,7 @@ class Engine final : public MinCutEngine {
       LF_HIP(hipMalloc(&planes_, sizeof(unsigned long long) * wp_.ps * WPLANES));
       LF_HIP(hipMemsetAsync(planes_, 0, sizeof(unsigned long long) * wp_.ps * WPLANES, s_));
     }
-    LF_HIP(hipHostMalloc(&host_small_, 64, hipHostMallocDefault));
+    LF_HIP(hipHostMalloc(&host_small_, 128, hipHostMallocDefault));
     LF_HIP(hipEventCreate(&ev_[0]));
     LF_HIP(hipEventCreate(&ev_[1]));
     LF_HIP(hipEventCreate(&evb_[0]));
@@ -2677,7 +2677,9 @@ class Engine final : public MinCutEngine {
   }
 
   // working state (res, ex, zero mailbox) from the base network, with range checks
-  void load_working() {
+  // defer_check: the capacity checks are taken from the first global
+  // relabel's read instead of a synchronisation of their own
+  void load_working(bool defer_check = false) {
     KP<T> p = params();
     int* err = reinterpret_cast<int*>(dev_small_);
     unsigned long long* mx = reinterpret_cast<unsigned long long*>(dev_small_ + 32);
@@ -2693,10 +2695,23 @@ class Engine final : public MinCutEngine {
     hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 0, mx);
     hipLaunchKernelGGL((k_reduce<true>), dim3(1), dim3(NTHREADS), 0, s_, part_, lb, 2, 1, mx + 1);
     LF_HIP(hipGetLastError());
+    if (defer_check) {
+      load_check_pending_ = true;
+      return;
+    }
     LF_HIP(hipMemcpyAsync(host_small_, dev_small_, 64, hipMemcpyDeviceToHost, s_));
     LF_HIP(hipStreamSynchronize(s_));
-    const int e = reinterpret_cast<int*>(host_small_)[0];
-    const unsigned long long* m = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(host_small_) + 32);
+    check_load(host_small_);
+  }
+
+  // Capacity checks of a load from a host copy of dev_small_[0, 64) taken
+  // after k_load and its reductions: the error word at 0 (bit 1: negative
+  // pairwise capacity) and the largest node and arc capacities at 32 and 40.
+  // Until the first global relabel only the counter slot, the tile counter and
+  // the active count (16, 24, 48) change.
+  void check_load(const void* hs) {
+    const int e = static_cast<const int*>(hs)[0];
+    const unsigned long long* m = reinterpret_cast<const unsigned long long*>(static_cast<const char*>(hs) + 32);
     if (e & 2) fail(LF_ERR_NONSUBMODULAR, "negative pairwise capacity");
     max_red_ = m[0];
     max_alpha_ = m[1];
@@ -2842,30 +2857,34 @@ class Engine final : public MinCutEngine {
     bfs_time_pending_ = false;
     fill_many({{u32(tiles_ptr()), 0u, 2}, {u32(bvisits_), 0u, 16}});
   }
+  // One read for the statistics: the small device words (the counter slot,
+  // which holds the last reduction, and the processed-tile counter) into
+  // host_small_[0, 64) and the BFS visit counters into [64, 128).
   void end_stats() {
     stats.tile_nodes = grid_[variant_].nt;
     stats.bytes_per_node = 2 * ((long long)K * sizeof(T) + sizeof(T) + 4);
     stats.bfs_bytes_per_visit = (long long)K * sizeof(T) + 4LL * K + 4;
-    LF_HIP(hipMemcpy(host_small_, tiles_ptr(), sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    stats.tiles_pushed = (long long)*reinterpret_cast<unsigned long long*>(host_small_);
-    unsigned long long v[8];
-    LF_HIP(hipMemcpy(v, bvisits_, sizeof(v), hipMemcpyDeviceToHost));
+    char* hb = static_cast<char*>(host_small_);
+    LF_HIP(hipMemcpyAsync(hb, dev_small_, 64, hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipMemcpyAsync(hb + 64, bvisits_, 64, hipMemcpyDeviceToHost, s_));
+    LF_HIP(hipStreamSynchronize(s_));
+    const unsigned long long* w = reinterpret_cast<const unsigned long long*>(hb);
+    stats.tiles_pushed = (long long)w[3];  // tiles_ptr(): dev_small_ + 24
     stats.bfs_visits = 0;
-    for (unsigned long long x : v) stats.bfs_visits += (long long)x;
+    for (int i = 0; i < 8; ++i) stats.bfs_visits += (long long)w[8 + i];
   }
 
   void run(const lf_tuning& tun) override {
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     begin_stats();
-    load_working();
+    load_working(true);
     solve_state(tun);
     KP<T> p = params();
     hipLaunchKernelGGL((k_flow<T>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, p, red0_, part_);
     hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, counter_ptr());
-    flow = (long long)read_counter();
-    LF_HIP(hipStreamSynchronize(s_));
-    end_stats();
+    end_stats();  // its read brings the flow too (the counter slot, dev_small_ + 16)
+    flow = (long long)reinterpret_cast<const unsigned long long*>(host_small_)[2];
     stats.seconds = std::chrono::duration<double>(clk::now() - t0).count();
     solved = true;
     pf_solved = false;
@@ -3692,11 +3711,17 @@ class Engine final : public MinCutEngine {
                        p_.ntiles, wl_[cur_], wl_cnt_ + wl_k_);
     hipLaunchKernelGGL((k_reduce<false>), dim3(1), dim3(NTHREADS), 0, s_, part_, red_blocks(), 1, 0, active_ptr());
     LF_HIP(hipGetLastError());
-    // one read: the border counter of the last BFS launch (slot 0) up to the
-    // active count (slot 4)
-    LF_HIP(hipMemcpyAsync(host_small_, counter_ptr(), 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s_));
+    // one read of the small device words: the border counter of the last BFS
+    // launch (slot 0) up to the active count (slot 4), and the checks of a
+    // load that deferred them
+    LF_HIP(hipMemcpyAsync(host_small_, dev_small_, 64, hipMemcpyDeviceToHost, s_));
     LF_HIP(hipStreamSynchronize(s_));
-    const unsigned long long* hs = reinterpret_cast<const unsigned long long*>(host_small_);
+    if (load_check_pending_) {
+      load_check_pending_ = false;
+      check_load(host_small_);
+    }
+    const unsigned long long* hs =
+        reinterpret_cast<const unsigned long long*>(static_cast<const char*>(host_small_) + 16);
     take_bfs_time();
     if (deferred) *exact = hs[0] == 0;
     tiles_seen_ = hs[1];
@@ -3748,6 +3773,7 @@ class Engine final : public MinCutEngine {
   bool bfs_time_pending_ = false;
   unsigned long long* bvisits_ = nullptr;   // exact-BFS node visits, one slot per XCD
   unsigned long long max_red_ = 0, max_alpha_ = 0;
+  bool load_check_pending_ = false;  // a load's capacity checks wait for the next relabel's read
   const bool trace_ = std::getenv("LF_TRACE") != nullptr;
   const bool trace_pf_ = std::getenv("LF_TRACE_PF") != nullptr;
   bool tv_mode_ = false;  // TV network: regions that a mean cut leaves whole are final
