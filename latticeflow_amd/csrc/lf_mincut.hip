@@ -2848,7 +2848,7 @@ class Engine final : public MinCutEngine {
     run_tiled([&](const BP& q) {
       launch_flood(q);
       ++stats.flood_launches;
-    });
+    }, 0, false, nullptr, false, FLOOD_FIRST_BATCH);
     stats.flood_seconds += std::chrono::duration<double>(clk::now() - tf).count();
   }
 
@@ -3166,7 +3166,7 @@ class Engine final : public MinCutEngine {
         run_tiled([&](const BP& q) {
           launch_flood(q);
           ++stats.flood_launches;
-        });
+        }, 0, false, nullptr, false, FLOOD_FIRST_BATCH);
         hipLaunchKernelGGL((k_pf_stats_impl<1>), dim3(red_blocks()), dim3(NTHREADS), 0, s_, pf, (const int*)reach_, 4);
         hipLaunchKernelGGL((k_pf_split), dim3(blocks_for(nreg)), dim3(NTHREADS), 0, s_, pf, nreg, 4, 5);
         hipLaunchKernelGGL((k_pf_move), dim3(blocks_for(N_)), dim3(NTHREADS), 0, s_, pf, (const int*)reach_, 5);
@@ -3454,6 +3454,10 @@ class Engine final : public MinCutEngine {
   // 32-lambda 1024^2 TV path (time-balanced 4.15 s; ratio 2: 3.93 s, 3: 3.92 s,
   // 6: 4.23 s) and on the 256^3 unthresholded noise volume (0.57 s; 3: 0.50 s,
   // 8: 0.50 s).  LF_PUSH_RELABEL_RATIO overrides the ratio (for studies).
+  // S_min floods settle within about four launches on 2048^2 segmentation and
+  // 256^2 grids (launches over settled tiles exit at their flag test, a few
+  // microseconds each): the first check comes after four, one read instead of two.
+  static constexpr int FLOOD_FIRST_BATCH = 4;
   static double push_relabel_ratio() {
     static const double r = env_double("LF_PUSH_RELABEL_RATIO", 3.0);
     return r;
@@ -3531,11 +3535,13 @@ class Engine final : public MinCutEngine {
   // the border counter with its own next read.
   // timed: the device time of the launch batches (hipEvents around each
   // batch, read once the batch is synchronised) goes to stats.bfs_device_seconds
+  // first_batch: launches before the first check (then twice as many, up to 8)
   template <class F>
-  bool run_tiled(F&& launch, long long cap = 0, bool resume = false, bool* deferred = nullptr, bool timed = false) {
+  bool run_tiled(F&& launch, long long cap = 0, bool resume = false, bool* deferred = nullptr, bool timed = false,
+                 int first_batch = 2) {
     if (!resume) fill_many({{u32(bflags_[bcur_]), 1u, bntiles_}, {u32(bflags_[bcur_ ^ 1]), 0u, bntiles_}});
     long long launched = 0;
-    for (int batch = 2;; batch = std::min(batch * 2, 8)) {
+    for (int batch = first_batch;; batch = std::min(batch * 2, 8)) {
       if (cap > 0 && launched >= cap) return false;  // stopped before the fixpoint
       if (timed) LF_HIP(hipEventRecord(evb_[0], s_));
       for (int b = 0; b < batch; ++b) {
