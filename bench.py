@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # (rounds per relabel, local iterations, tile shape, relabel launches), fixed
 # so that the line and its rocprofv3 profile run the same schedule;
 # --autotune runs the search instead.
-DEFAULT_TUNING = {"seg8": (4, 8, 1, 2), "blobs26": (4, 2, 1, 2), "grid4": (6, 4, 3, 2), "noise26": (0, 0, 1, 0)}
+DEFAULT_TUNING = {"seg8": (4, 8, 1, 2), "blobs26": (4, 2, 1, 2), "grid4": (12, 4, 2, 2), "noise26": (0, 0, 1, 0)}
 
 
 def make_workload(name, shape):

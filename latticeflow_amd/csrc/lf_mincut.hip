@@ -2628,10 +2628,7 @@ class Engine final : public MinCutEngine {
     if (!err.empty()) fail(LF_ERR_STATE, std::string("state check after ") + where + ": " + err);
   }
 
-  // Push tiling of a solve: the tuning's tile shape, else the shorter tiles
-  // (G1) for 2D int64 networks, whose residual planes take twice the LDS (the
-  // 32-lambda 1024^2 TV path measured 13.4 s there against 21.5 s on G0),
-  // else G0.
+  // Push tiling of a solve: the tuning's tile shape, else G0.
   int variant_for(const lf_tuning& tun) const {
     if (tun.tile_shape >= 1 && tun.tile_shape <= 3) return tun.tile_shape - 1;
     return 0;
@@ -3834,7 +3831,10 @@ std::unique_ptr<MinCutEngine> make_k(const LatticeGeom& g, int cap_bytes) {
 // Push tilings (G0..G2) and exact-pass tile (B) of the radius-1 neighbourhoods.
 struct Tiles2d1 {
   using G0 = Geo<1, 16, 32, 0, 1, 1>;
-  using G1 = Geo<1, 8, 64, 0, 1, 1>;
+  // one node per thread: the shortest rounds, for small lattices whose rounds
+  // are one tile's latency (256^2 grid: 10.5 us per round against 19.6 us on
+  // 32x32 tiles)
+  using G1 = Geo<1, 16, 16, 0, 1, 1>;
   using G2 = Geo<1, 32, 32, 0, 1, 1>;
   using B = BGeo<1, 64, 64, 0, 1, 1>;
 };
