@@ -664,11 +664,24 @@ __global__ __launch_bounds__(NTHREADS) void k_flags_to_list(const int* __restric
   if (t < ntiles && flags[t]) list[atomicAdd(cnt, 1u)] = t;
 }
 
+// Lattice coordinates of node v in 32-bit unsigned arithmetic (a short
+// sequence where 64-bit division by the run-time extents is a long software
+// one): the tiled engines keep N * sizeof(element) below 2^32, so v < 2^30.
+__device__ inline void node_coords(long long v, int H, int W, int& z, int& y, int& x) {
+  const unsigned u = (unsigned)v;
+  const unsigned r = u / (unsigned)W;
+  x = (int)(u - r * (unsigned)W);
+  const unsigned zz = r / (unsigned)H;
+  y = (int)(r - zz * (unsigned)H);
+  z = (int)zz;
+}
+
 // ---- mailbox delivery (before any exact pass) --------------------------------
 
 template <int K, class G, typename T, class DIR>
 __device__ inline void deliver_node(const KP<T>& p, long long v) {
-  const int x = (int)(v % p.W), y = (int)((v / p.W) % p.H), z = (int)(v / ((long long)p.W * p.H));
+  int z, y, x;
+  node_coords(v, p.H, p.W, z, y, x);
   const int lx = x % G::TW, ly = y % G::TH, lz = z % G::TD;
   if (lx >= G::RX && lx < G::TW - G::RX && ly >= G::RY && ly < G::TH - G::RY && lz >= G::RZ &&
       lz < G::TD - G::RZ)
@@ -966,7 +979,8 @@ __global__ __launch_bounds__(NTHREADS) void k_count_active(KP<T> p, long long* _
   for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < p.N; v += (long long)gridDim.x * NTHREADS) {
     if (p.ex[v] > 0 && p.h[v] < INF_H) {
       ++cnt;
-      const int x = (int)(v % p.W), y = (int)((v / p.W) % p.H), z = (int)(v / ((long long)p.W * p.H));
+      int z, y, x;
+      node_coords(v, p.H, p.W, z, y, x);
       flag_neighbor_tile<G, T>(p, z, y, x, p.flag_out);
     }
   }
@@ -2160,7 +2174,8 @@ template <int K, typename T>
 __global__ __launch_bounds__(NTHREADS) void k_pf_cutarcs(PFP f, KP<T> p) {
   const long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x;
   if (v >= f.N) return;
-  const int x = (int)(v % p.W), y = (int)((v / p.W) % p.H), z = (int)(v / ((long long)p.W * p.H));
+  int z, y, x;
+  node_coords(v, p.H, p.W, z, y, x);
   const int k = f.reg[v];
   MaskT<K> cut = 0;
 #pragma unroll
