@@ -40,7 +40,8 @@ typedef struct {
   int32_t rounds_per_relabel; /* push/relabel rounds between exact global relabels */
   int32_t local_iters;        /* push/relabel iterations per tile per round, in LDS */
   int32_t max_cycles;         /* cap on global-relabel cycles (0 = none) */
-  int32_t tile_shape;         /* push-tile shape: 0 = default (shape 1), 1..3 = the shapes of
+  int32_t tile_shape;         /* push-tile shape: 0 = default (shape 1; shape 2 for parametric /
+                                 TV networks on 2D radius-1 lattices), 1..3 = the shapes of
                                  lf_autotune; for radius-1 kernels 2D 16x32 / 16x16 / 32x32,
                                  3D 4x4x16 / 2x8x16 / 4x8x8 (4x8x32 / 2x8x32 / 4x16x16 with
                                  6 directions) */
