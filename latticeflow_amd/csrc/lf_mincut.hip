@@ -825,6 +825,13 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
   constexpr int LCAP = B::NT / 2;
   __shared__ int s_d[B::NH];
   __shared__ MT s_m[B::NT];
+  // 2-D tiles of byte masks (at most 8 directions): bit j of s_in[v] says the
+  // in-tile neighbour v + dir j has an arc to v, so a relaxation reads one
+  // byte instead of every neighbour's mask (2048^2 segmentation 1.257 -> 1.207
+  // ms, 1024^2 TV path 3.70 -> 3.38 s; on 16^3 tiles of 32-bit masks its 16 KB
+  // cost more residency than the reads it saves)
+  constexpr bool INM = B::TD == 1 && sizeof(MT) == 1;
+  __shared__ MT s_in[INM ? B::NT : 1];
   __shared__ unsigned short s_f[2][LCAP];
   __shared__ unsigned int s_q[B::NT / 32];
   __shared__ int s_n[3];
@@ -889,6 +896,19 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
     const int dv = s_d[hi];
     d0[j] = dv;
     const MT m = s_m[t];
+    if constexpr (INM) {
+      const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
+      MT im = 0;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int dz = DIR::z(p.dirs, i), dy = DIR::y(p.dirs, i), dx = DIR::x(p.dirs, i);
+        const int uz = z + dz, uy = y + dy, ux = x + dx;
+        const bool intile = uz >= 0 && uz < B::TD && uy >= 0 && uy < B::TH && ux >= 0 && ux < B::TW;
+        const MT mu = s_m[intile ? t + (dz * B::TH + dy) * B::TW + dx : t];
+        if (intile && (mu & ((MT)1 << ((i + K / 2) % K)))) im |= (MT)1 << i;
+      }
+      s_in[t] = im;
+    }
     if (dv == 0 || m == 0) continue;
     int best = dv;
 #pragma unroll
@@ -934,15 +954,21 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
       __threadfence_block();
       const int hv = bhalo<B>(v);
       const int nd = s_d[hv] + 1;
+      const MT im = INM ? s_in[v] : (MT)0;
       const int z = v / (B::TH * B::TW), y = (v / B::TW) % B::TH, x = v % B::TW;
 #pragma unroll
       for (int j = 0; j < K; ++j) {
         const int dz = DIR::z(p.dirs, j), dy = DIR::y(p.dirs, j), dx = DIR::x(p.dirs, j);
-        const int uz = z + dz, uy = y + dy, ux = x + dx;
-        if (uz < 0 || uz >= B::TD || uy < 0 || uy >= B::TH || ux < 0 || ux >= B::TW) continue;
         const int u = v + (dz * B::TH + dy) * B::TW + dx;
-        // arc u -> v is u's direction rev(j)
-        if (!(s_m[u] & ((MT)1 << ((j + K / 2) % K)))) continue;
+        // the in-tile neighbours u = v + dir j with an arc u -> v (u's
+        // direction rev(j))
+        if constexpr (INM) {
+          if (!((im >> j) & 1)) continue;
+        } else {
+          const int uz = z + dz, uy = y + dy, ux = x + dx;
+          if (uz < 0 || uz >= B::TD || uy < 0 || uy >= B::TH || ux < 0 || ux >= B::TW) continue;
+          if (!(s_m[u] & ((MT)1 << ((j + K / 2) % K)))) continue;
+        }
         const int hu = hv + (dz * B::HH + dy) * B::HW + dx;
         if (s_d[hu] > nd && atomicMin(&s_d[hu], nd) > nd) {
           // claim u for the next list
