@@ -353,8 +353,12 @@ int lf_autotune(lf_handle* h, int reps, lf_tuning* best) {
       }
       return tot;
     };
-    // stage 1: the adaptive schedule and five fixed ones, on every tile shape
-    const lf_tuning sched[] = {{0, 0, 0, 0, 0}, {4, 4, 0, 0, 0}, {4, 8, 0, 0, 0}, {6, 4, 0, 0, 0}, {4, 2, 0, 0, 0}, {8, 2, 0, 0, 0}};
+    // stage 1: the adaptive schedule and seven fixed ones, on every tile shape
+    // (one round of one iteration per relabel suits networks that are nearly
+    // solved at the start, 12 rounds of 4 small lattices whose rounds are
+    // latency)
+    const lf_tuning sched[] = {{0, 0, 0, 0, 0}, {4, 4, 0, 0, 0}, {4, 8, 0, 0, 0},  {6, 4, 0, 0, 0},
+                               {4, 2, 0, 0, 0}, {8, 2, 0, 0, 0}, {1, 1, 0, 0, 0}, {12, 4, 0, 0, 0}};
     lf_tuning best_t = h->tuning;
     double best_s = 1e300;
     for (int shape = 1; shape <= 3; ++shape)
