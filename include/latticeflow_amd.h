@@ -250,7 +250,7 @@ int lf_tv_path_guided(const char *kernel, int64_t nx, int64_t ny, const double *
  * device, then level sets that join in closed form and split where a minimum
  * cut says so, down to lambda = 0.  Every new level set is checked by one
  * minimum cut (more where it splits): level sets of at most host_max nodes on
- * the host, larger ones on the device (0: 32768; negative: all on the
+ * the host, larger ones on the device (0: 2048; negative: all on the
  * device).  lf_tvpath_eval then gives the path at any lambdas in
  * [0, lambda_max] (getRegularizationPath, tv_solver.hpp:181-244, traced per
  * pixel on the device): slice l for lambdas[l], negative lambdas count as 0,

@@ -27,7 +27,11 @@ namespace lfamd {
 namespace {
 
 constexpr int kTraceThreads = 256;
-constexpr int64_t kDefaultHostMax = 32768;
+// Level sets of at most this many nodes are checked on the host: a device
+// check costs 5-8 ms (a whole-lattice region solve), a host cut of a level
+// set of a few thousand nodes as much (128^2 / 256^2 Voronoi paths: 2.7 / 30.2 s
+// at 32768, 2.2 / 13.8 s at 2048, 2.9 / 18.6 s at 512)
+constexpr int64_t kDefaultHostMax = 2048;
 
 class DeviceChecker : public RegionChecker {
  public:
