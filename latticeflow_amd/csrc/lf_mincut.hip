@@ -3500,16 +3500,19 @@ class Engine final : public MinCutEngine {
     return v ? std::atof(v) : dflt;
   }
   // Adaptive schedules choose the push rounds between two relabels so that
-  // pushing takes about three times as long as the relabel before it: on the
-  // 32-lambda 1024^2 TV path (time-balanced 4.15 s; ratio 2: 3.93 s, 3: 3.92 s,
-  // 6: 4.23 s) and on the 256^3 unthresholded noise volume (0.57 s; 3: 0.50 s,
-  // 8: 0.50 s).  LF_PUSH_RELABEL_RATIO overrides the ratio (for studies).
+  // pushing takes about 2.5 times as long as the relabel before it.  Measured
+  // with the ratio at 1 / 2 / 2.5 / 3 / 5: the 512-lattice batch 69.1 / 66.9 /
+  // 67.7 / 73.0 / 85.3 ms; the 32-lambda 1024^2 TV path (16x32 tiles) 4.15 s
+  // time-balanced, 3.93 s at 2, 3.92 s at 3, 4.23 s at 6, and 3.34-3.51 s at
+  // 2-3 on 16x16 tiles; the 256^3 unthresholded noise volume 0.57 s balanced,
+  // 0.52-0.53 s at 2-2.5, 0.50 s at 3.  LF_PUSH_RELABEL_RATIO overrides the
+  // ratio (for studies).
   // S_min floods settle within about four launches on 2048^2 segmentation and
   // 256^2 grids (launches over settled tiles exit at their flag test, a few
   // microseconds each): the first check comes after four, one read instead of two.
   static constexpr int FLOOD_FIRST_BATCH = 4;
   static double push_relabel_ratio() {
-    static const double r = env_double("LF_PUSH_RELABEL_RATIO", 3.0);
+    static const double r = env_double("LF_PUSH_RELABEL_RATIO", 2.5);
     return r;
   }
   void ensure_guide_buffers(int layers) {
