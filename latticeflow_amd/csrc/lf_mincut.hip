@@ -820,10 +820,15 @@ __device__ inline void flag_around(const BP& p, int tile) {
 template <int K, class B, typename MT, class DIR>
 __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restrict__ mask, int* __restrict__ hgl) {
   // a list holds up to half the tile; a step whose list overflowed walks the
-  // listed-node bits instead (every claimed node has its bit set), so the
-  // lists cost half the LDS and more tiles are resident per CU
+  // listed nodes instead, so the lists cost half the LDS and more tiles are
+  // resident per CU.
+  // s_d[q] = 2 label + 1 while node q is not listed, 2 label while it is: one
+  // atomicOr releases a node and returns its label, one atomicMin lowers a
+  // neighbour's label and says whether it must be listed (it was not, and the
+  // word changed), where a separate listed-bit array took a release, a fence
+  // and a read, and a lowering plus a claim
   constexpr int LCAP = B::NT / 2;
-  __shared__ int s_d[B::NH];
+  __shared__ unsigned s_d[B::NH];
   __shared__ MT s_m[B::NT];
   // 2-D tiles of byte masks (at most 8 directions): bit j of s_in[v] says the
   // in-tile neighbour v + dir j has an arc to v, so a relaxation reads one
@@ -833,7 +838,6 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
   constexpr bool INM = B::TD == 1 && sizeof(MT) == 1;
   __shared__ MT s_in[INM ? B::NT : 1];
   __shared__ unsigned short s_f[2][LCAP];
-  __shared__ unsigned int s_q[B::NT / 32];
   __shared__ int s_n[3];
   __shared__ int s_flag;
   const int tile = tile_of_block(p.nblocks);
@@ -877,14 +881,13 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
 #pragma unroll
     for (int k = 0; k < NHI; ++k) {
       const int q = threadIdx.x + k * BTHREADS;
-      if (q < B::NH) s_d[q] = hin[k] ? hv[k] : INF_H;
+      if (q < B::NH) s_d[q] = 2u * (unsigned)(hin[k] ? hv[k] : INF_H) + 1u;
     }
 #pragma unroll
     for (int j = 0; j < B::NPT; ++j) {
       const int t = threadIdx.x + j * BTHREADS;
       s_m[t] = in[j] ? mv[j] : (MT)0;
     }
-    for (int w = threadIdx.x; w < B::NT / 32; w += BTHREADS) s_q[w] = 0;
   }
   int d0[B::NPT];
   __syncthreads();
@@ -893,7 +896,7 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
   for (int j = 0; j < B::NPT; ++j) {
     const int t = threadIdx.x + j * BTHREADS;
     const int hi = bhalo<B>(t);
-    const int dv = s_d[hi];
+    const int dv = (int)(s_d[hi] >> 1);
     d0[j] = dv;
     const MT m = s_m[t];
     if constexpr (INM) {
@@ -914,12 +917,13 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
 #pragma unroll
     for (int i = 0; i < K; ++i)
       if (m & ((MT)1 << i)) {
-        const int dn = s_d[hi + (DIR::z(p.dirs, i) * B::HH + DIR::y(p.dirs, i)) * B::HW + DIR::x(p.dirs, i)] + 1;
+        const int dn =
+            (int)(s_d[hi + (DIR::z(p.dirs, i) * B::HH + DIR::y(p.dirs, i)) * B::HW + DIR::x(p.dirs, i)] >> 1) + 1;
         best = dn < best ? dn : best;
       }
     if (best < dv) {
-      atomicMin(&s_d[hi], best);
-      atomicOr(&s_q[t >> 5], 1u << (t & 31));
+      // only the owner writes its node in this pass: lowered and listed
+      s_d[hi] = 2u * (unsigned)best;
       const int pos = atomicAdd(&s_n[0], 1);
       if (pos < LCAP) s_f[0][pos] = (unsigned short)t;
     }
@@ -941,19 +945,18 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
     for (int q = threadIdx.x; q < lim; q += BTHREADS) {
       int v;
       if (scan) {
-        // a bit set during this step is a claim for the next one: taking it
-        // now as well only repeats an idempotent relaxation
+        // a node listed during this step is a claim for the next one: taking
+        // it now as well only repeats an idempotent relaxation
         v = q;
-        if (!((s_q[v >> 5] >> (v & 31)) & 1u)) continue;
+        if (s_d[bhalo<B>(v)] & 1u) continue;
       } else {
         v = s_f[cur][q];
       }
-      // release v's list bit before reading its label: a later drop of the
-      // label then re-lists v instead of being lost
-      atomicAnd(&s_q[v >> 5], ~(1u << (v & 31)));
-      __threadfence_block();
+      // release v and read its label in one atomic: a later drop of the label
+      // then lists v again instead of being lost
       const int hv = bhalo<B>(v);
-      const int nd = s_d[hv] + 1;
+      const int nd = (int)(atomicOr(&s_d[hv], 1u) >> 1) + 1;
+      const unsigned e_nd = 2u * (unsigned)nd;  // label nd, listed
       const MT im = INM ? s_in[v] : (MT)0;
       const int z = v / (B::TH * B::TW), y = (v / B::TW) % B::TH, x = v % B::TW;
 #pragma unroll
@@ -970,10 +973,10 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
           if (!(s_m[u] & ((MT)1 << ((j + K / 2) % K)))) continue;
         }
         const int hu = hv + (dz * B::HH + dy) * B::HW + dx;
-        if (s_d[hu] > nd && atomicMin(&s_d[hu], nd) > nd) {
-          // claim u for the next list
-          const unsigned int bit = 1u << (u & 31);
-          if (!(atomicOr(&s_q[u >> 5], bit) & bit)) {
+        if ((int)(s_d[hu] >> 1) > nd) {
+          const unsigned o = atomicMin(&s_d[hu], e_nd);
+          // this atomic listed u: it changed the word of a node not listed
+          if (o > e_nd && (o & 1u)) {
             const int pos = atomicAdd(&s_n[c_out], 1);
             if (pos < LCAP) s_f[cur ^ 1][pos] = (unsigned short)u;
           }
@@ -988,7 +991,7 @@ __global__ __launch_bounds__(BTHREADS) void k_bfs_tile(BP p, const MT* __restric
     const int t = threadIdx.x + j * BTHREADS;
     const int z = t / (B::TH * B::TW), y = (t / B::TW) % B::TH, x = t % B::TW;
     if (z >= ld || y >= lh || x >= lw) continue;
-    const int dv = s_d[bhalo<B>(t)];
+    const int dv = (int)(s_d[bhalo<B>(t)] >> 1);
     if (dv != d0[j]) {
       hgl[((long long)(z0 + z) * p.H + (y0 + y)) * p.W + (x0 + x)] = dv;
       if (bborder<B>(t)) border = true;
