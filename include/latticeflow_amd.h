@@ -46,7 +46,7 @@ typedef struct {
                                  3D 4x4x16 / 2x8x16 / 4x8x8 (4x8x32 / 2x8x32 / 4x16x16 with
                                  6 directions) */
   int32_t relabel_launches;   /* launches after which a global relabel between push rounds stops
-                                 (0 = default: 2 for min-cut solves, 16 for parametric / TV
+                                 (0 = default: 2 for min-cut solves, 4 for parametric / TV
                                  networks); the relabel that decides termination always runs
                                  to its fixpoint */
 } lf_tuning;

@@ -2801,12 +2801,14 @@ class Engine final : public MinCutEngine {
     // a short relabel's reach stay INF, which only idles their nodes until a
     // later relabel: min-cut networks with sinks everywhere measured fastest
     // at 2 launches (2048^2 segmentation 5.3 -> 2.2 ms, 256^3 blobs 31 -> 8.5
-    // ms), parametric / TV networks, whose flow travels far, at 16.  A min-cut
+    // ms), parametric / TV networks, whose flow travels farther, at 4 (the
+    // 32-lambda 1024^2 TV path on 16x16 tiles: 2 launches 3.75 s, 3-6 launches
+    // 3.08-3.29 s, 8 3.27-3.32 s, 16 3.29-3.35 s).  A min-cut
     // solve that keeps going is one whose flow travels far too: its reach
     // doubles every 4 cycles up to 64 launches (256^3 unthresholded noise
     // volume: 2.7 s at a fixed 2 launches, 0.66-0.85 s at 8-16).
     const bool fixed_cap = tun.relabel_launches > 0;
-    long long relabel_cap = fixed_cap ? tun.relabel_launches : (pf_mode_ ? 16 : 2);
+    long long relabel_cap = fixed_cap ? tun.relabel_launches : (pf_mode_ ? 4 : 2);
     bool need_exact = false, resume = false, push_timed_later = false;
     for (;;) {
       const auto tb = clk::now();
