@@ -574,8 +574,11 @@ def tvpath_record(a, full_line=False):
     f = W.tv_voronoi(seed=5, shape=shape)
     lams = W.tv_lambdas()
     net = TVNet.load()
-    for _ in range(a.warmup):
+    for i in range(a.warmup):
         _tv_path(f, lams, "Full2d_8")
+        if i == 0:
+            # the net's first forward pass loads its kernels (about 0.18 s once)
+            tv_path_learned(f, lams, "Full2d_8", net=net)
     times, ltimes = [], []
     R = Rl = st = gs = None
     for _ in range(max(a.steps, 1)):
