@@ -1903,12 +1903,13 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __
     hc[i] = 0;
   }
   __syncthreads();
-  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < f.N; v += (long long)gridDim.x * NTHREADS) {
-    const int k = f.reg[v];
-    RegionStat* t = &f.tab[k];
-    if (t->state != st) continue;
-    const long long x = MODE == 0 ? f.r[v] : MODE == 2 ? pred_r(f, v) : (long long)(side[v] != 0);
-    if (MODE == 1 && x == 0) continue;
+  // Each thread folds runs of RUN consecutive nodes in registers: neighbouring
+  // nodes mostly share a region, so the table takes one update per region
+  // change instead of one per node (per-node updates serialise the wave on
+  // one LDS slot).  Sums, extremes and counts are integers: the result does
+  // not depend on the order.
+  constexpr int RUN = 8;
+  auto put = [&](int k, long long sum, unsigned long long mn, unsigned long long mx, unsigned c) {
     int h = (int)(((unsigned)k * 2654435761u) >> 24) & (HSZ - 1);
     int slot = -1;
     for (int probe = 0; probe < 16; ++probe) {
@@ -1921,25 +1922,55 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_stats_impl(PFP f, const int* __
       h = (h + 1) & (HSZ - 1);
     }
     if (slot >= 0) {
-      if (MODE == 0) atomicAdd((unsigned long long*)&hs[slot], (unsigned long long)x);
+      if (MODE == 0) atomicAdd((unsigned long long*)&hs[slot], (unsigned long long)sum);
       if (MODE != 1) {
-        atomicMin(&hmn[slot], ord64(x));
-        atomicMax(&hmx[slot], ord64(x));
+        atomicMin(&hmn[slot], mn);
+        atomicMax(&hmx[slot], mx);
       }
-      atomicAdd(&hc[slot], 1u);
+      atomicAdd(&hc[slot], c);
     } else {
+      RegionStat* t = &f.tab[k];
       if (MODE == 0) {
-        add128(t, x);
-        atomicMin(&t->mn, ord64(x));
-        atomicMax(&t->mx, ord64(x));
-        atomicAdd(&t->cnt, 1ull);
+        add128(t, sum);
+        atomicMin(&t->mn, mn);
+        atomicMax(&t->mx, mx);
+        atomicAdd(&t->cnt, (unsigned long long)c);
       } else if (MODE == 2) {
-        atomicMin(&t->pmn, ord64(x));
-        atomicMax(&t->pmx, ord64(x));
+        atomicMin(&t->pmn, mn);
+        atomicMax(&t->pmx, mx);
       } else {
-        atomicAdd(&t->cnt1, 1ull);
+        atomicAdd(&t->cnt1, (unsigned long long)c);
       }
     }
+  };
+  const long long nrun = (f.N + RUN - 1) / RUN;
+  for (long long c = (long long)blockIdx.x * NTHREADS + threadIdx.x; c < nrun; c += (long long)gridDim.x * NTHREADS) {
+    const long long v0 = c * RUN, v1 = v0 + RUN < f.N ? v0 + RUN : f.N;
+    int ck = -1;
+    bool live = false;
+    long long sum = 0;
+    unsigned long long mn = ~0ull, mx = 0;
+    unsigned cnt = 0;
+    for (long long v = v0; v < v1; ++v) {
+      const int k = f.reg[v];
+      if (k != ck) {
+        if (cnt) put(ck, sum, mn, mx, cnt);
+        ck = k;
+        live = f.tab[k].state == st;
+        sum = 0;
+        mn = ~0ull;
+        mx = 0;
+        cnt = 0;
+      }
+      if (!live) continue;
+      const long long x = MODE == 0 ? f.r[v] : MODE == 2 ? pred_r(f, v) : (long long)(side[v] != 0);
+      if (MODE == 1 && x == 0) continue;
+      sum += x;
+      mn = ord64(x) < mn ? ord64(x) : mn;
+      mx = ord64(x) > mx ? ord64(x) : mx;
+      ++cnt;
+    }
+    if (cnt) put(ck, sum, mn, mx, cnt);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < HSZ; i += NTHREADS) {
