@@ -2027,13 +2027,11 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_hist(PFP f) {
     hc[i] = 0;
   }
   __syncthreads();
-  for (long long v = (long long)blockIdx.x * NTHREADS + threadIdx.x; v < f.N; v += (long long)gridDim.x * NTHREADS) {
-    const RegionStat& t = f.tab[f.reg[v]];
-    if (t.state != 1 || !t.fresh || t.hslot < 0) continue;
-    const long long lo = unord64(t.pmn), range = unord64(t.pmx) - lo + 1;
-    long long b = (long long)((double)(pred_r(f, v) - lo) * f.nbins / (double)range);
-    b = b < 0 ? 0 : (b >= f.nbins ? f.nbins - 1 : b);
-    const int key = t.hslot * f.nbins + (int)b;
+  // runs of RUN consecutive nodes counted in registers while their bin stays
+  // the same (smooth predictions: neighbours mostly share one), one table
+  // update per run instead of one per node
+  constexpr int RUN = 8;
+  auto put = [&](int key, unsigned c) {
     int h = (int)(((unsigned)key * 2654435761u) >> 23) & (HSZ - 1);
     int slot = -1;
     for (int probe = 0; probe < 16; ++probe) {
@@ -2045,8 +2043,29 @@ __global__ __launch_bounds__(NTHREADS) void k_pf_hist(PFP f) {
       }
       h = (h + 1) & (HSZ - 1);
     }
-    if (slot >= 0) atomicAdd(&hc[slot], 1u);
-    else atomicAdd(&f.hist[key], 1u);
+    if (slot >= 0) atomicAdd(&hc[slot], c);
+    else atomicAdd(&f.hist[key], c);
+  };
+  const long long nrun = (f.N + RUN - 1) / RUN;
+  for (long long c = (long long)blockIdx.x * NTHREADS + threadIdx.x; c < nrun; c += (long long)gridDim.x * NTHREADS) {
+    const long long v0 = c * RUN, v1 = v0 + RUN < f.N ? v0 + RUN : f.N;
+    int ckey = -1;
+    unsigned cnt = 0;
+    for (long long v = v0; v < v1; ++v) {
+      const RegionStat& t = f.tab[f.reg[v]];
+      if (t.state != 1 || !t.fresh || t.hslot < 0) continue;
+      const long long lo = unord64(t.pmn), range = unord64(t.pmx) - lo + 1;
+      long long b = (long long)((double)(pred_r(f, v) - lo) * f.nbins / (double)range);
+      b = b < 0 ? 0 : (b >= f.nbins ? f.nbins - 1 : b);
+      const int key = t.hslot * f.nbins + (int)b;
+      if (key != ckey) {
+        if (cnt) put(ckey, cnt);
+        ckey = key;
+        cnt = 0;
+      }
+      ++cnt;
+    }
+    if (cnt) put(ckey, cnt);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < HSZ; i += NTHREADS)
